@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the Pauli-vector hot path on one MI355X.
+
+Workload: the 14-qubit (4^14 = 268,435,456-entry) density matrix in the full
+qubit Pauli basis driven by the seeded 60-layer noisy brickwork circuit.
+One step = the 13 two-qubit PTM applications of one even+odd brickwork layer
+pair, bonds (0,1), (2,3), ..., (12,13), then (1,2), ..., (11,12): every
+target position from outermost to innermost.  The state stays resident in
+HBM; PTMs travel in the kernel arguments.
+
+Printed JSON line (stdout):
+  value      = 2-qubit PTM applications per second (float64 state)
+  roofline   = the 2-qubit apply kernel against 8 TB/s HBM; algorithmic bytes
+               per launch = (N_in + N_out) * 8 + 16*16*8 (PTM)
+  cpu_baseline = the float64 numpy oracle (same einsum structure as the
+               reference backend) applying the same 2-qubit PTMs to a
+               14-qubit state on this host's cores, bounded sample.
+Extra fields: float32 rate, per-position kernel GB/s, 1-qubit rate, the
+rate of whole circuit layers, and the diagonal readout time.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--cpu-budget', type=float, default=20.0,
+                    help='seconds of numpy work for the CPU baseline sample')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--qubits', type=int, default=14)
+    return ap.parse_args()
+
+
+def timed(fn, reps, stream):
+    """Average seconds per call of fn() over reps calls, HIP events on the
+    stream the kernels are launched on."""
+    import torch
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(reps):
+        fn()
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / 1e3 / reps
+
+
+def load_traffic():
+    """HBM bytes per launch of the 2-qubit apply kernel from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    path = os.path.join(ROOT, 'profiles', 'apply_ptm_2q_14q_f64_pmc.txt')
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith('traffic_bytes_per_launch:'):
+                return float(line.split(':', 1)[1])
+    return None
+
+
+def cpu_baseline(n, pairs, ptm, budget):
+    import numpy as np
+    from oracle.pauli_vector import PauliVectorNumpy
+    from oracle import bases as ob
+    cores = len(os.sched_getaffinity(0))
+    o = PauliVectorNumpy([ob.general(2)] * n, force=True)
+    o._data = np.random.default_rng(0).standard_normal([4] * n)
+    times, done = [], []
+    t_all = time.perf_counter()
+    for qs in pairs:
+        t = time.perf_counter()
+        o.apply_ptm(ptm, *qs)
+        times.append(time.perf_counter() - t)
+        done.append(qs)
+        if time.perf_counter() - t_all > budget:
+            break
+    per = sum(times) / len(times)
+    return {
+        'value': 1.0 / per, 'unit': '2q-PTM apps/s', 'cores': cores, 'kind': 'port',
+        'sample': '%d 2-qubit PTM applications on the %d-qubit float64 state, targets %s, '
+                  'numpy einsum(optimize=True) as in the reference backend; '
+                  'OPENBLAS_NUM_THREADS=%s' % (len(done), n, done,
+                                               os.environ.get('OPENBLAS_NUM_THREADS', 'unset')),
+        'seconds_per_app': [round(x, 3) for x in times],
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path '
+              '(--gpus %d requested)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import quantumsim_amd as qa
+    from quantumsim_amd import bases
+    from quantumsim_amd.workloads import noisy_layers, layer_slices
+
+    n = args.qubits
+    N = 4 ** n
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    circuit, meta = noisy_layers(n, 60, seed=1, pairing='brickwork')
+    twoq = [(p, q) for p, q in circuit[:len(circuit)] if len(q) == 2]
+    step_gates = [g for g in twoq[:(n // 2) + (n - 1) // 2]]
+    assert len(step_gates) == n - 1
+    pairs = [q for _, q in step_gates]
+
+    res = {}
+    for dtype, es in (('float64', 8), ('float32', 4)):
+        v = qa.PauliVectorHIP([bases.general(2)] * n, force=True, dtype=dtype)
+        v._data.copy_(torch.randn(N, dtype=v.dtype, device=v.device))
+
+        def step():
+            for p, q in step_gates:
+                v.apply_ptm(p, *q)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sec_step = timed(step, args.steps, stream)
+        wall = time.perf_counter() - t0
+        apps = len(step_gates) / sec_step
+        per_launch = sec_step / len(step_gates)
+        alg_bytes = 2 * N * es + 256 * es
+        res[dtype] = dict(apps=apps, sec_step=sec_step, per_launch=per_launch,
+                          gbs=alg_bytes / per_launch / 1e9, wall=wall, alg_bytes=alg_bytes)
+        # per-position kernel rates
+        pos = {}
+        for label, qs in (('outer(0,1)', (0, 1)), ('middle(6,7)', (n // 2 - 1, n // 2)),
+                          ('inner(%d,%d)' % (n - 2, n - 1), (n - 2, n - 1))):
+            p = step_gates[0][0]
+            t = timed(lambda: v.apply_ptm(p, *qs), 10, stream)
+            pos[label] = round(alg_bytes / t / 1e9, 1)
+        res[dtype]['positions_GBs'] = pos
+        # one-qubit applications
+        oneq = [(p, q) for p, q in circuit[:3 * n] if len(q) == 1][:n]
+        t1 = timed(lambda: [v.apply_ptm(p, *q) for p, q in oneq], 5, stream) / len(oneq)
+        res[dtype]['oneq_apps'] = 1.0 / t1
+        res[dtype]['oneq_GBs'] = (2 * N * es + 16 * es) / t1 / 1e9
+        # whole circuit layers (1q Haar + CZ + damping per layer)
+        sl = layer_slices(meta, len(circuit))
+        two_layers = circuit[sl[0].start:sl[1].stop]
+        tl = timed(lambda: [v.apply_ptm(p, *q) for p, q in two_layers], 3, stream) / 2
+        res[dtype]['layer_ms'] = tl * 1e3
+        res[dtype]['layer_gates'] = len(two_layers) / 2
+        # diagonal readout (gather of 2^n entries, float64 out)
+        td = timed(lambda: v.diagonal(get_data=False), 10, stream)
+        res[dtype]['diag_us'] = td * 1e6
+        del v
+        torch.cuda.empty_cache()
+        log('%s: %.1f 2q apps/s, %.1f GB/s per launch, positions %s, 1q %.1f apps/s, '
+            'layer %.2f ms, diag %.1f us' % (dtype, apps, res[dtype]['gbs'], pos,
+                                              res[dtype]['oneq_apps'], tl * 1e3, td * 1e6))
+
+    r64 = res['float64']
+    out = {
+        'metric': '2q-PTM apps/s + HBM GB/s (14q DM); batched fwd+grad circuits/s vs plain',
+        'value': round(r64['apps'], 2),
+        'unit': '2q-PTM applications/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(r64['sec_step'] * 1e3, 4),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (seeded noisy brickwork circuit, random state)',
+        'config': {'workload': '14-qubit noisy circuit, plain method: 2-qubit PTM applications '
+                               'on the 4^14-entry Pauli vector, all 13 brickwork bonds per step',
+                   'qubits': n, 'entries': N, 'gates_per_step': len(step_gates)},
+        'roofline': {'bound': 'hbm', 'achieved': round(r64['gbs'], 1), 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': round(r64['gbs'] / HBM_PEAK_GBS, 4),
+                     'traffic': load_traffic(),
+                     'kernel': 'k_apply_small<double,16,*> (2-qubit apply)',
+                     'alg_bytes_per_launch': r64['alg_bytes'],
+                     'avg_launch_ms': round(r64['per_launch'] * 1e3, 4)},
+        'f32': {'apps_per_s': round(res['float32']['apps'], 2),
+                'GBs': round(res['float32']['gbs'], 1),
+                'frac': round(res['float32']['gbs'] / HBM_PEAK_GBS, 4)},
+        'positions_GBs': {'f64': r64['positions_GBs'],
+                          'f32': res['float32']['positions_GBs']},
+        'oneq': {'f64_apps_per_s': round(r64['oneq_apps'], 1),
+                 'f64_GBs': round(r64['oneq_GBs'], 1),
+                 'f32_apps_per_s': round(res['float32']['oneq_apps'], 1)},
+        'layer': {'gates_per_layer': r64['layer_gates'],
+                  'f64_ms': round(r64['layer_ms'], 3),
+                  'f32_ms': round(res['float32']['layer_ms'], 3)},
+        'diagonal_us': {'f64': round(r64['diag_us'], 1),
+                        'f32': round(res['float32']['diag_us'], 1)},
+    }
+    if not args.no_cpu:
+        log('cpu baseline (numpy oracle, budget %.0f s)...' % args.cpu_budget)
+        cpu_pairs = [(0, 1), (n // 2 - 1, n // 2), (n - 2, n - 1)]
+        out['cpu_baseline'] = cpu_baseline(n, cpu_pairs, step_gates[0][0], args.cpu_budget)
+    else:
+        out['cpu_baseline'] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,120 @@
+/*
+ * libqsv — MI355X (gfx950) kernels for Pauli-vector density matrices.
+ *
+ * C ABI only: plain pointers, sizes and a hipStream_t passed as void*.
+ * Every call is stream-ordered and asynchronous; none allocates device
+ * memory (callers hand in workspaces sized by the *_workspace_bytes
+ * queries).  Return value 0 = success, negative = error; the message of
+ * the last error on the calling thread is returned by qsv_last_error().
+ *
+ * Layout convention (quantumsim/pauli_vectors/cuda.py:170 and
+ * quantumsim/pauli_vectors/numpy.py:55-64): a Pauli vector over n qudits is
+ * a dense C-order tensor of shape dim_pauli[0..n-1]; qudit 0 is the most
+ * significant (slowest) axis.  A k-qudit PTM is a C-order tensor of shape
+ * (out_0..out_{k-1}, in_0..in_{k-1}) (quantumsim/operations/operation.py:476).
+ */
+#ifndef QSV_H
+#define QSV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types of a Pauli vector on the device */
+#define QSV_F32 0
+#define QSV_F64 1
+
+/* error codes */
+#define QSV_OK 0
+#define QSV_EINVAL (-1)   /* bad argument: shape, index, dtype, size     */
+#define QSV_EHIP (-2)     /* a HIP runtime call failed                   */
+#define QSV_ENOTSUP (-3)  /* valid request outside what the kernels do   */
+
+#define QSV_MAX_RANK 32    /* qudits per Pauli vector                    */
+#define QSV_MAX_TARGETS 8  /* qudits per PTM                             */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* qsv_last_error(void);
+
+/* ABI version (major*100 + minor). */
+int qsv_abi_version(void);
+
+/* Bytes of device workspace qsv_apply_ptm needs for this PTM shape
+ * (0 when the PTM and its offset tables travel inside the kernel
+ * arguments).  dims_in/dims_out are the k target dims. */
+int64_t qsv_apply_ptm_workspace_bytes(int dtype, int k, const int32_t* dims_in_t,
+                                      const int32_t* dims_out_t);
+
+/* out = PTM applied to `in` on the qudits `targets` (k of them, in PTM axis
+ * order; any order, no duplicates).
+ *   dims[rank]      : input tensor shape (Pauli dims per qudit)
+ *   dims_out_t[k]   : PTM output dims for the k targets (the tensor shape
+ *                     of `out` is dims with dims[targets[j]] := dims_out_t[j])
+ *   ptm_f64         : host pointer, C-order (out..., in...) float64 PTM
+ *   workspace       : device memory of qsv_apply_ptm_workspace_bytes()
+ *                     bytes (may be NULL when that is 0)
+ * `in` and `out` must not overlap.
+ * Replaces PauliVectorNumpy.apply_ptm (quantumsim/pauli_vectors/numpy.py:50-64)
+ * and PauliVectorCuda._apply_{single,two}_qubit_ptm
+ * (quantumsim/pauli_vectors/cuda.py:151-277 -> primitives.cu:15-89). */
+int qsv_apply_ptm(int dtype, const void* in, void* out, int rank,
+                  const int64_t* dims, int k, const int32_t* targets,
+                  const int32_t* dims_out_t, const double* ptm_f64,
+                  void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Same contraction with the PTM already on the device, C-order
+ * (out..., in...), in the state's element type (used when PTMs are built
+ * on the device, e.g. from trainable parameters, and under graph capture).
+ * `workspace` holds the offset tables (qsv_apply_ptm_workspace_bytes). */
+int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
+                      const int64_t* dims, int k, const int32_t* targets,
+                      const int32_t* dims_out_t, const void* ptm_dev,
+                      void* workspace, int64_t workspace_bytes, void* stream);
+
+/* out = in[np.ix_(idx_0, ..., idx_{rank-1})] with an index of -1 producing
+ * 0.  The index lists live in device memory as int32: idx_dev[offsets[r] +
+ * j] is the j-th index of axis r, for j < out_dims[r]; offsets (host) has
+ * rank entries.  in_dtype -> out_dtype conversion is exact (f32 -> f64) or
+ * a plain copy.  Bit-exact gather.
+ * Replaces the multitake kernel (quantumsim/pauli_vectors/primitives.cu:92-130,
+ * host side cuda.py:307-336). */
+int qsv_multitake(int in_dtype, int out_dtype, const void* in, void* out,
+                  int rank, const int64_t* in_dims, const int64_t* out_dims,
+                  const int32_t* idx_dev, const int32_t* offsets,
+                  void* stream);
+
+/* Bytes of device workspace qsv_axis_sum needs. */
+int64_t qsv_axis_sum_workspace_bytes(int64_t n_outer, int d, int64_t n_inner);
+
+/* out_f64[j] = sum_{o,i} in[o, j, i] for `in` viewed as [n_outer, d, n_inner],
+ * accumulated in float64, deterministic order; out_f64 is a device pointer
+ * of d doubles.  d <= 64.  With d = 1 this is the trace of a diagonal;
+ * with the diagonal viewed around one qudit it is meas_prob.
+ * Replaces pycuda's ReductionKernel sum_along_axis
+ * (quantumsim/pauli_vectors/cuda.py:46-52, 366-375) and the host sums of
+ * numpy.py:82-84, 97-107. */
+int qsv_axis_sum(int dtype, const void* in, int64_t n_outer, int d,
+                 int64_t n_inner, double* out_f64, void* workspace,
+                 int64_t workspace_bytes, void* stream);
+
+/* data[i] = data[i] * s for i < n, the product taken in float64 and rounded
+ * once to the element type.  Replaces the scaling step of renormalize
+ * (quantumsim/pauli_vectors/numpy.py:109-117, cuda.py:386-395). */
+int qsv_scale(int dtype, void* data, int64_t n, double s, void* stream);
+
+/* ---- diagnostics (host only, no GPU work) ------------------------------
+ * Checks the library's 32-bit magic-number division against hardware
+ * division for `n_divisors` divisors (1, 2, ..., plus all powers of 2, 3, 4,
+ * 9 and random values) times a fixed set of numerators including 0 and
+ * 2^32-1.  Returns the number of mismatches (0 = pass).  No counterpart in
+ * the reference; it lets CPU tests pin the index arithmetic every kernel
+ * uses. */
+int64_t qsv_selftest_fastdiv(int32_t n_divisors, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSV_H */

@@ -1,0 +1,85 @@
+"""Seeded synthetic circuits of the benchmark configurations.
+
+A circuit is a list of ``(ptm, qubits)`` with float64 PTMs in the
+``general(2)`` basis (layout (out..., in...)), ready for ``apply_ptm``.
+
+* ``noisy_layers``: per layer a Haar-random 1-qubit unitary on every qubit,
+  CZ on a set of disjoint pairs, then amplitude + phase damping of every
+  qubit for a 20 ns gate with T1 ~ U(20, 40) us, T2 ~ U(10, T1) us drawn once
+  per qubit.
+  - ``pairing='random'``: a fresh random disjoint pairing per layer (6-qubit,
+    40-layer case);
+  - ``pairing='brickwork'``: nearest-neighbour CZ on a line, bonds (0,1),
+    (2,3), ... on even layers and (1,2), (3,4), ... on odd layers (14-qubit,
+    60-layer case).
+"""
+import numpy as np
+
+from .algebra.algebra import kraus_to_ptm
+from .bases import general
+from .operations import qubits as gates
+
+__all__ = ['haar_unitary', 'noisy_layers', 'layer_slices', 'damping_ptm', 'circuit_bytes']
+
+_B1 = (general(2),)
+GATE_TIME = 20e-9
+
+
+def haar_unitary(d, rng):
+    """Haar-distributed unitary: QR of a complex Ginibre matrix with the
+    phase correction of Mezzadri, 'How to generate random matrices from the
+    classical compact groups' (2007)."""
+    z = (rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    ph = np.diagonal(r) / np.abs(np.diagonal(r))
+    return q * ph
+
+
+def damping_ptm(t1, t2, t=GATE_TIME):
+    g, lam = gates.t1t2_rates(t, t1, t2)
+    return gates.amp_phase_damping(g, lam).ptm
+
+
+def noisy_layers(n_qubits, n_layers, seed, pairing='brickwork'):
+    """Returns (circuit, meta) -- see the module docstring."""
+    rng = np.random.default_rng(seed)
+    t1 = rng.uniform(20e-6, 40e-6, size=n_qubits)
+    t2 = np.array([rng.uniform(10e-6, t1[q]) for q in range(n_qubits)])
+    damp = [damping_ptm(t1[q], t2[q]) for q in range(n_qubits)]
+    cz = gates.cphase(np.pi).ptm
+    circuit = []
+    layer_starts = []
+    for layer in range(n_layers):
+        layer_starts.append(len(circuit))
+        for q in range(n_qubits):
+            u = haar_unitary(2, rng)
+            circuit.append((kraus_to_ptm(u[None], _B1, _B1), (q,)))
+        if pairing == 'random':
+            perm = rng.permutation(n_qubits)
+            pairs = [tuple(sorted((int(perm[2 * i]), int(perm[2 * i + 1]))))
+                     for i in range(n_qubits // 2)]
+        elif pairing == 'brickwork':
+            start = layer % 2
+            pairs = [(q, q + 1) for q in range(start, n_qubits - 1, 2)]
+        else:
+            raise ValueError(pairing)
+        for pair in pairs:
+            circuit.append((cz, pair))
+        for q in range(n_qubits):
+            circuit.append((damp[q], (q,)))
+    meta = {'t1': t1, 't2': t2, 'layer_starts': layer_starts, 'seed': seed,
+            'n_qubits': n_qubits, 'n_layers': n_layers, 'pairing': pairing}
+    return circuit, meta
+
+
+def layer_slices(meta, n_gates):
+    starts = list(meta['layer_starts']) + [n_gates]
+    return [slice(a, b) for a, b in zip(starts[:-1], starts[1:])]
+
+
+def circuit_bytes(circuit, n_qubits, elem_bytes, dim=4):
+    """Algorithmic HBM bytes of applying ``circuit`` gate by gate to an
+    n-qubit full-basis vector: (N_in + N_out) * s per gate (PTM bytes are
+    negligible and ignored)."""
+    n = dim ** n_qubits
+    return len(circuit) * 2 * n * elem_bytes
