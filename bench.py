@@ -172,6 +172,16 @@ def main():
             'layer %.2f ms, diag %.1f us' % (dtype, apps, res[dtype]['gbs'], pos,
                                               res[dtype]['oneq_apps'], tl * 1e3, td * 1e6))
 
+    # device-to-device copy ceiling: same bytes moved as one apply launch
+    src = torch.empty(N, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    tc = timed(lambda: dst.copy_(src), 10, stream)
+    copy_gbs = 2 * N * 8 / tc / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    log('copy ceiling %.1f GB/s' % copy_gbs)
+
     r64 = res['float64']
     out = {
         'metric': '2q-PTM apps/s + HBM GB/s (14q DM); batched fwd+grad circuits/s vs plain',
@@ -201,6 +211,7 @@ def main():
         'layer': {'gates_per_layer': r64['layer_gates'],
                   'f64_ms': round(r64['layer_ms'], 3),
                   'f32_ms': round(res['float32']['layer_ms'], 3)},
+        'copy_ceiling_GBs': round(copy_gbs, 1),
         'diagonal_us': {'f64': round(r64['diag_us'], 1),
                         'f32': round(res['float32']['diag_us'], 1)},
     }

@@ -113,6 +113,19 @@ int qsv_scale(int dtype, void* data, int64_t n, double s, void* stream);
  * uses. */
 int64_t qsv_selftest_fastdiv(int32_t n_divisors, uint64_t seed);
 
+/* CPU replay of qsv_apply_ptm for a vector of at most 2^24 entries: the
+ * same planner picks the kernel path the GPU call would take for `dtype`
+ * (reported in *path_out: 1 direct, 2 LDS-tiled, 3 table), then every
+ * thread's address arithmetic is executed on the host in float64 with each
+ * global and LDS access checked against its buffer and every output entry
+ * checked to be written exactly once.  in/out/ptm are host float64 arrays.
+ * Returns QSV_EINVAL with a message on any violation.  No counterpart in
+ * the reference; it lets CPU tests pin the kernels' indexing before GPU use. */
+int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, int rank,
+                          const int64_t* dims, int k, const int32_t* targets,
+                          const int32_t* dims_out_t, const double* ptm_f64,
+                          int32_t* path_out);
+
 #ifdef __cplusplus
 }
 #endif

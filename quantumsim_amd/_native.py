@@ -49,6 +49,9 @@ _SIGNATURES = {
                                     ctypes.c_int64, _vp, _vp, ctypes.c_int64, _vp]),
     'qsv_scale': (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_double, _vp]),
     'qsv_selftest_fastdiv': (ctypes.c_int64, [ctypes.c_int32, ctypes.c_uint64]),
+    'qsv_emulate_apply_ptm': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int,
+                                             _c_i64p, ctypes.c_int, _c_i32p, _c_i32p, _c_f64p,
+                                             _c_i32p]),
 }
 
 _lib = None

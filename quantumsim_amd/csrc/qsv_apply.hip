@@ -17,6 +17,7 @@
 #include "qsv_common.h"
 
 #include <algorithm>
+#include <stdlib.h>
 #include <string.h>
 #include <type_traits>
 #include <vector>
@@ -44,6 +45,20 @@ template <typename T>
 struct VT<T, 1> {
   typedef T t;
 };
+
+// Global accesses of the streaming kernels.  NT = nontemporal (streaming)
+// hint, used when the vector is too large to stay cache-resident between
+// consecutive gates.
+template <bool NT, typename V>
+__device__ __forceinline__ V gload(const V* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void gstore(V* p, const V& v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
 
 // --- operand sources.  p(o, i) is only called with o < dout, i < din.
 // PTM padded to CH x CH (zeros) and both offset tables, by value.
@@ -84,8 +99,8 @@ struct TableOp {
   int32_t padded;
 };
 
-__device__ __forceinline__ void orbit_base(const Geom& g, uint32_t m, int64_t& bin,
-                                           int64_t& bout) {
+__host__ __device__ __forceinline__ void orbit_base(const Geom& g, uint32_t m, int64_t& bin,
+                                                    int64_t& bout) {
   bin = 0;
   bout = 0;
 #pragma unroll
@@ -106,7 +121,7 @@ __device__ __forceinline__ void orbit_base(const Geom& g, uint32_t m, int64_t& b
 
 // ---------------------------------------------------------------- small
 // din, dout <= CH: all D_in operands live in registers, outputs stream out.
-template <typename T, int CH, int VEC, class Op>
+template <typename T, int CH, int VEC, bool NT, class Op>
 __global__ void __launch_bounds__(kBlock)
     k_apply_small(const T* __restrict__ in, T* __restrict__ out, const Geom g, const Op op) {
   typedef typename VT<T, VEC>::t V;
@@ -119,7 +134,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int i = 0; i < CH; ++i) {
     if (i < g.din)
-      v[i] = *reinterpret_cast<const V*>(in + bin + op.oin(i));
+      v[i] = gload<NT>(reinterpret_cast<const V*>(in + bin + op.oin(i)));
     else
       v[i] = V(0);
   }
@@ -131,7 +146,7 @@ __global__ void __launch_bounds__(kBlock)
       for (int i = 0; i < CH; ++i) {
         if (Op::kPadded || i < g.din) acc += op.p(o, i) * v[i];
       }
-      *reinterpret_cast<V*>(out + bout + op.oout(o)) = acc;
+      gstore<NT>(reinterpret_cast<V*>(out + bout + op.oout(o)), acc);
     }
   }
 }
@@ -179,13 +194,186 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+
+// ------------------------------------------------------------ LDS tiled
+// For targets near the innermost axis the orbit entries of neighbouring
+// lanes are strided, so direct loads touch many cache lines per
+// instruction.  Here a block owns a "tile": up to 256 orbits whose entries
+// form a few long contiguous runs (one run per combination of the targets
+// outside the tile's region).  The runs are copied into LDS with coalesced
+// 16-byte loads, each thread contracts one orbit out of LDS, and the output
+// runs are written back coalesced.  LDS rows are padded by one element per
+// 16 so that orbits 16 elements apart fall into different banks.
+constexpr int kMaxRuns = 16;
+
+struct TileGeom {
+  uint32_t n_tiles;
+  int32_t ntg;  // tile-index groups (global), innermost first
+  FastDiv tdiv[kMaxGroups];
+  int64_t ts_in[kMaxGroups], ts_out[kMaxGroups];
+  int32_t runs_in, runs_out;  // runs per tile (combos of targets outside the region)
+  int32_t L_in, L_out;        // run lengths, elements
+  int64_t run_in[kMaxRuns], run_out[kMaxRuns];  // run offsets from the tile base
+  int32_t orbits;             // orbits per tile (<= 256)
+  FastDiv pdiv;               // orbits per region block
+  int32_t R_in, R_out;        // region block sizes
+  int32_t nrg;                // bystander groups inside the region
+  FastDiv rdiv[kMaxGroups];
+  int32_t rs_in[kMaxGroups], rs_out[kMaxGroups];
+  int32_t din, dout;
+  int32_t slot_in[16], slot_out[16];  // LDS offsets of the PTM indices
+  int32_t in_place;
+};
+
+__host__ __device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
+
+// global bases of tile `m`
+__host__ __device__ __forceinline__ void tile_base(const TileGeom& tg, uint32_t m, int64_t& gin,
+                                                   int64_t& gout) {
+  gin = 0;
+  gout = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxGroups; ++j) {
+    if (j >= tg.ntg) break;
+    if (j == tg.ntg - 1) {
+      gin += int64_t(m) * tg.ts_in[j];
+      gout += int64_t(m) * tg.ts_out[j];
+    } else {
+      const uint32_t q = fdiv(m, tg.tdiv[j]);
+      const uint32_t r = m - q * tg.tdiv[j].d;
+      gin += int64_t(r) * tg.ts_in[j];
+      gout += int64_t(r) * tg.ts_out[j];
+      m = q;
+    }
+  }
+}
+
+// LDS bases (before padding) of the orbit owned by thread `tid` of a tile
+__host__ __device__ __forceinline__ void tile_orbit_base(const TileGeom& tg, int tid, int& bin,
+                                                         int& bout) {
+  const uint32_t xc = fdiv(uint32_t(tid), tg.pdiv);
+  uint32_t rb = uint32_t(tid) - xc * tg.pdiv.d;
+  bin = int(xc) * tg.R_in;
+  bout = int(xc) * tg.R_out;
+#pragma unroll
+  for (int j = 0; j < kMaxGroups; ++j) {
+    if (j >= tg.nrg) break;
+    if (j == tg.nrg - 1) {
+      bin += int(rb) * tg.rs_in[j];
+      bout += int(rb) * tg.rs_out[j];
+    } else {
+      const uint32_t q = fdiv(rb, tg.rdiv[j]);
+      const uint32_t r = rb - q * tg.rdiv[j].d;
+      bin += int(r) * tg.rs_in[j];
+      bout += int(r) * tg.rs_out[j];
+      rb = q;
+    }
+  }
+}
+
+template <typename T, int VEC, bool NT, int CH, int ORB, class Op>
+__global__ void __launch_bounds__(kBlock)
+    k_apply_lds(const T* __restrict__ in, T* __restrict__ out, const TileGeom tg, const Op op) {
+  typedef typename VT<T, VEC>::t V;
+  extern __shared__ __align__(16) unsigned char qsv_smem[];
+  T* lin = reinterpret_cast<T*>(qsv_smem);
+  T* lout = tg.in_place ? lin : lin + lds_pad(tg.runs_in * tg.L_in) + 1;
+  const int tid = threadIdx.x;
+
+  int64_t gin, gout;
+  tile_base(tg, blockIdx.x, gin, gout);
+
+  // stage the input runs
+  for (int a = 0; a < tg.runs_in; ++a) {
+    const T* src = in + gin + tg.run_in[a];
+    const int base = a * tg.L_in;
+    for (int e = tid * VEC; e < tg.L_in; e += kBlock * VEC) {
+      const V val = gload<NT>(reinterpret_cast<const V*>(src + e));
+      if constexpr (VEC == 1) {
+        lin[lds_pad(base + e)] = val;
+      } else {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) lin[lds_pad(base + e + c)] = val[c];
+      }
+    }
+  }
+  __syncthreads();
+
+  // contract ORB orbits per thread: orbits tid, tid + 256, ...
+  int bout[ORB];
+  T v[ORB][CH];
+#pragma unroll
+  for (int r = 0; r < ORB; ++r) {
+    const int j = tid + r * kBlock;
+    bout[r] = 0;
+    if (j < tg.orbits) {
+      int bin;
+      tile_orbit_base(tg, j, bin, bout[r]);
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        v[r][i] = i < tg.din ? lin[lds_pad(bin + tg.slot_in[i])] : T(0);
+    }
+  }
+  if (tg.in_place) __syncthreads();
+#pragma unroll
+  for (int r = 0; r < ORB; ++r) {
+    if (tid + r * kBlock < tg.orbits) {
+#pragma unroll
+      for (int o = 0; o < CH; ++o) {
+        if (o < tg.dout) {
+          T acc = T(0);
+#pragma unroll
+          for (int i = 0; i < CH; ++i) {
+            if (Op::kPadded || i < tg.din) acc += op.p(o, i) * v[r][i];
+          }
+          lout[lds_pad(bout[r] + tg.slot_out[o])] = acc;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // write the output runs
+  for (int a = 0; a < tg.runs_out; ++a) {
+    T* dst = out + gout + tg.run_out[a];
+    const int base = a * tg.L_out;
+    for (int e = tid * VEC; e < tg.L_out; e += kBlock * VEC) {
+      V val;
+      if constexpr (VEC == 1) {
+        val = lout[lds_pad(base + e)];
+      } else {
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) val[c] = lout[lds_pad(base + e + c)];
+      }
+      gstore<NT>(reinterpret_cast<V*>(dst + e), val);
+    }
+  }
+}
+
 // ------------------------------------------------------------ host side
+// Combined in+out bytes above which global accesses use the nontemporal
+// hint (QSV_NT_BYTES overrides it).
+static int64_t nt_threshold_bytes() {
+  static int64_t v = [] {
+    const char* e = getenv("QSV_NT_BYTES");
+    return e ? int64_t(atoll(e)) : (int64_t(256) << 20);
+  }();
+  return v;
+}
+
 struct Plan {
   Geom g;
   int64_t n_in, n_out;
   int vec;
+  bool nt;  // streaming (nontemporal) global accesses
   int din, dout;
   std::vector<int64_t> off_in, off_out;
+  // per-axis description kept for the LDS-tiled planner
+  int rank, k;
+  int32_t targets[QSV_MAX_TARGETS];
+  int64_t dims_in[QSV_MAX_RANK], dims_out[QSV_MAX_RANK];
+  int64_t sin[QSV_MAX_RANK], sout[QSV_MAX_RANK];
+  bool is_target[QSV_MAX_RANK];
 };
 
 static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int32_t* targets,
@@ -226,6 +414,19 @@ static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int3
   }
   p.n_in = acc_in;
   p.n_out = acc_out;
+  // vectors that cannot stay resident in the last-level caches between
+  // gates are streamed with nontemporal accesses
+  p.nt = (acc_in + acc_out) * dtype_size(dtype) > nt_threshold_bytes();
+  p.rank = rank;
+  p.k = k;
+  for (int j = 0; j < k; ++j) p.targets[j] = targets[j];
+  for (int q = 0; q < rank; ++q) {
+    p.dims_in[q] = dims[q];
+    p.dims_out[q] = dout_full[q];
+    p.sin[q] = sin[q];
+    p.sout[q] = sout[q];
+    p.is_target[q] = is_target[q];
+  }
 
   int64_t din = 1, dout = 1;
   for (int j = 0; j < k; ++j) {
@@ -303,13 +504,272 @@ static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int3
   return QSV_OK;
 }
 
+
+// Byte threshold on the contiguous run below the innermost target under
+// which the LDS-tiled kernel replaces the direct one (QSV_LDS_RUN_BYTES
+// overrides it for tuning).
+static int64_t lds_run_threshold() {
+  static int64_t v = [] {
+    const char* e = getenv("QSV_LDS_RUN_BYTES");
+    return e ? int64_t(atoll(e)) : int64_t(512);
+  }();
+  return v;
+}
+
+struct LdsPlan {
+  TileGeom tg;
+  int vec;
+  bool nt;
+  int ch;   // register tile width: 4 or 16
+  int orb;  // orbits per thread: 1, 2 or 4
+  size_t lds_bytes;
+};
+
+// Plans the LDS-tiled variant; returns false when the direct kernel is the
+// better (or only) choice.
+// Orbits per LDS tile: about 8-16 KB of staged input (measured best on
+// MI355X for 14-qubit vectors), at most 4 orbits per thread.
+// QSV_LDS_ORBITS overrides it for tuning.
+constexpr int kLdsMaxOrbits = 4 * kBlock;
+static int64_t lds_tile_orbits(int64_t dmax, int elem_bytes) {
+  static const int64_t forced = [] {
+    const char* e = getenv("QSV_LDS_ORBITS");
+    return e ? int64_t(atoll(e)) : int64_t(0);
+  }();
+  // measured optimum: 16 KB tiles, except 1-qubit float64 PTMs (8 KB)
+  const int64_t tile_bytes = (elem_bytes == 8 && dmax <= 4) ? 8192 : 16384;
+  int64_t x = forced > 0 ? forced : tile_bytes / (dmax * elem_bytes);
+  if (dmax > 4 && x > 2 * kBlock) x = 2 * kBlock;  // 16-wide register tiles: <= 2 orbits
+  return x < 16 ? int64_t(16) : (x > kLdsMaxOrbits ? int64_t(kLdsMaxOrbits) : x);
+}
+
+static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
+  if (p.din > 16 || p.dout > 16) return false;
+  const int64_t cap = lds_tile_orbits(std::max(p.din, p.dout), elem_bytes);
+  int t_in = -1;
+  for (int j = 0; j < p.k; ++j) t_in = std::max(t_in, int(p.targets[j]));
+  int64_t Z = 1;
+  for (int q = t_in + 1; q < p.rank; ++q) Z *= p.dims_in[q];
+  if (Z * elem_bytes >= lds_run_threshold()) return false;
+
+  TileGeom& tg = lp.tg;
+  memset(&tg, 0, sizeof(tg));
+  // grow the region [c0, rank) upward while a tile stays <= 256 orbits
+  int c0 = t_in;
+  int64_t P = Z;  // orbits per region block
+  while (c0 > 0) {
+    const int c = c0 - 1;
+    const int64_t np = p.is_target[c] ? P : P * p.dims_in[c];
+    if (np > cap) break;
+    P = np;
+    c0 = c;
+  }
+  int64_t R_in = 1, R_out = 1;
+  for (int q = c0; q < p.rank; ++q) {
+    R_in *= p.dims_in[q];
+    R_out *= p.dims_out[q];
+  }
+  // chunk of the bystander axis right above the region
+  int64_t Tx = 1;
+  if (c0 > 0) {
+    const int64_t d = p.dims_in[c0 - 1];  // a bystander (targets were absorbed)
+    for (int64_t t = d; t >= 1; --t)
+      if (d % t == 0 && P * t <= cap) {
+        Tx = t;
+        break;
+      }
+  }
+  const int64_t orbits = Tx * P;
+  if (orbits < 1 || orbits > cap) return false;
+  tg.orbits = int32_t(orbits);
+  tg.pdiv = make_fastdiv(uint32_t(P));
+  tg.R_in = int32_t(R_in);
+  tg.R_out = int32_t(R_out);
+  tg.L_in = int32_t(Tx * R_in);
+  tg.L_out = int32_t(Tx * R_out);
+
+  // runs: combinations of targets outside the region (ascending axis order)
+  int outer[QSV_MAX_TARGETS];
+  int nouter = 0;
+  for (int q = 0; q < c0; ++q)
+    if (p.is_target[q]) outer[nouter++] = q;
+  int64_t runs_in = 1, runs_out = 1;
+  for (int j = 0; j < nouter; ++j) {
+    runs_in *= p.dims_in[outer[j]];
+    runs_out *= p.dims_out[outer[j]];
+  }
+  if (runs_in > kMaxRuns || runs_out > kMaxRuns) return false;
+  tg.runs_in = int32_t(runs_in);
+  tg.runs_out = int32_t(runs_out);
+  for (int64_t a = 0; a < runs_in; ++a) {
+    int64_t r = a, off = 0;
+    for (int j = nouter - 1; j >= 0; --j) {
+      off += (r % p.dims_in[outer[j]]) * p.sin[outer[j]];
+      r /= p.dims_in[outer[j]];
+    }
+    tg.run_in[a] = off;
+  }
+  for (int64_t a = 0; a < runs_out; ++a) {
+    int64_t r = a, off = 0;
+    for (int j = nouter - 1; j >= 0; --j) {
+      off += (r % p.dims_out[outer[j]]) * p.sout[outer[j]];
+      r /= p.dims_out[outer[j]];
+    }
+    tg.run_out[a] = off;
+  }
+
+  // LDS offset of every PTM index: run * L + offset inside the region
+  auto slot = [&](int64_t i, bool is_out) {
+    int64_t r = i, run = 0, run_mul = 1, off = 0;
+    // digits of i, last target fastest
+    int64_t digit[QSV_MAX_TARGETS];
+    for (int j = p.k - 1; j >= 0; --j) {
+      const int q = p.targets[j];
+      const int64_t d = is_out ? p.dims_out[q] : p.dims_in[q];
+      digit[j] = r % d;
+      r /= d;
+    }
+    for (int jo = nouter - 1; jo >= 0; --jo) {
+      const int q = outer[jo];
+      int j = 0;
+      while (p.targets[j] != q) ++j;
+      run += digit[j] * run_mul;
+      run_mul *= is_out ? p.dims_out[q] : p.dims_in[q];
+    }
+    for (int j = 0; j < p.k; ++j) {
+      const int q = p.targets[j];
+      if (q >= c0) off += digit[j] * (is_out ? p.sout[q] : p.sin[q]);
+    }
+    return run * (is_out ? tg.L_out : tg.L_in) + off;
+  };
+  tg.din = p.din;
+  tg.dout = p.dout;
+  for (int i = 0; i < p.din; ++i) tg.slot_in[i] = int32_t(slot(i, false));
+  for (int o = 0; o < p.dout; ++o) tg.slot_out[o] = int32_t(slot(o, true));
+
+  // bystander groups inside the region (innermost first)
+  {
+    int ng = 0;
+    int q = p.rank - 1;
+    while (q >= c0) {
+      if (p.is_target[q]) {
+        --q;
+        continue;
+      }
+      int64_t size = 1;
+      const int64_t s_in = p.sin[q], s_out = p.sout[q];
+      while (q >= c0 && !p.is_target[q]) {
+        size *= p.dims_in[q];
+        --q;
+      }
+      if (size == 1) continue;
+      tg.rdiv[ng] = make_fastdiv(uint32_t(size));
+      tg.rs_in[ng] = int32_t(s_in);
+      tg.rs_out[ng] = int32_t(s_out);
+      ++ng;
+    }
+    tg.nrg = ng;
+  }
+
+  // tile-index groups: bystander axes above the region; the chunked axis
+  // contributes dims / Tx at stride Tx * s
+  {
+    int ng = 0;
+    int64_t n_tiles = 1;
+    int q = c0 - 1;
+    bool first = true;
+    while (q >= 0) {
+      if (p.is_target[q]) {
+        --q;
+        continue;
+      }
+      int64_t size = 1;
+      int64_t s_in = p.sin[q], s_out = p.sout[q];
+      while (q >= 0 && !p.is_target[q]) {
+        int64_t d = p.dims_in[q];
+        if (first && q == c0 - 1) {
+          d /= Tx;
+          s_in *= Tx;
+          s_out *= Tx;
+        }
+        size *= d;
+        --q;
+      }
+      first = false;
+      if (size == 1) continue;
+      if (size >= (int64_t(1) << 32)) return false;
+      tg.tdiv[ng] = make_fastdiv(uint32_t(size));
+      tg.ts_in[ng] = s_in;
+      tg.ts_out[ng] = s_out;
+      n_tiles *= size;
+      ++ng;
+    }
+    tg.ntg = ng;
+    if (n_tiles >= (int64_t(1) << 31)) return false;
+    tg.n_tiles = uint32_t(n_tiles);
+  }
+
+  bool square = true;
+  for (int j = 0; j < p.k; ++j)
+    square = square && p.dims_in[p.targets[j]] == p.dims_out[p.targets[j]];
+  tg.in_place = square ? 1 : 0;
+  const int want = elem_bytes == 8 ? 2 : 4;
+  lp.vec = (R_in % want == 0 && R_out % want == 0) ? want : 1;
+  const int64_t n_in = int64_t(tg.runs_in) * tg.L_in, n_out = int64_t(tg.runs_out) * tg.L_out;
+  const int64_t elems = (n_in + n_in / 16 + 1) + (square ? 0 : n_out + n_out / 16 + 1);
+  lp.lds_bytes = size_t(elems * elem_bytes);
+  lp.nt = false;  // measured: the staged runs are faster with normal caching
+  lp.ch = std::max(p.din, p.dout) <= 4 ? 4 : 16;
+  lp.orb = int((orbits + kBlock - 1) / kBlock);
+  if (lp.orb == 3) lp.orb = 4;
+  if (lp.ch == 16 && lp.orb > 2) return false;
+  if (lp.lds_bytes > 64 * 1024) return false;
+  return true;
+}
+
+template <typename T, int VEC, int CH, int ORB, class Op>
+static int launch_lds(const LdsPlan& lp, const Op& op, const void* in, void* out,
+                      hipStream_t s) {
+  if (lp.tg.n_tiles == 0) return QSV_OK;
+  hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, Op>), dim3(lp.tg.n_tiles),
+                     dim3(kBlock), lp.lds_bytes, s, static_cast<const T*>(in),
+                     static_cast<T*>(out), lp.tg, op);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+template <typename T, int VEC, int CH, class Op>
+static int dispatch_lds_orb(const LdsPlan& lp, const Op& op, const void* in, void* out,
+                            hipStream_t s) {
+  if constexpr (CH == 4) {
+    if (lp.orb == 4) return launch_lds<T, VEC, CH, 4>(lp, op, in, out, s);
+  }
+  if (lp.orb == 2) return launch_lds<T, VEC, CH, 2>(lp, op, in, out, s);
+  return launch_lds<T, VEC, CH, 1>(lp, op, in, out, s);
+}
+
+template <typename T, int CH, class Op>
+static int dispatch_lds(const LdsPlan& lp, const Op& op, const void* in, void* out,
+                        hipStream_t s) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (lp.vec == 2) return dispatch_lds_orb<T, 2, CH>(lp, op, in, out, s);
+  } else {
+    if (lp.vec == 4) return dispatch_lds_orb<T, 4, CH>(lp, op, in, out, s);
+  }
+  return dispatch_lds_orb<T, 1, CH>(lp, op, in, out, s);
+}
+
 template <typename T, int CH, int VEC, class Op>
 static int launch_small(const Plan& p, const Op& op, const void* in, void* out,
                         hipStream_t s) {
   if (p.g.n_threads == 0) return QSV_OK;
   const dim3 grid(ceil_div_u32(p.g.n_threads, kBlock));
-  hipLaunchKernelGGL((k_apply_small<T, CH, VEC, Op>), grid, dim3(kBlock), 0, s,
-                     static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
+  if (p.nt)
+    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, true, Op>), grid, dim3(kBlock), 0, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
+  else
+    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, false, Op>), grid, dim3(kBlock), 0, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
 }
@@ -338,6 +798,8 @@ static int run_inline(const Plan& p, const double* ptm, const void* in, void* ou
   memset(&op, 0, sizeof(op));
   for (int o = 0; o < p.dout; ++o)
     for (int i = 0; i < p.din; ++i) op.ptm[o * CH + i] = T(ptm[int64_t(o) * p.din + i]);
+  LdsPlan lp;
+  if (make_lds_plan(p, sizeof(T), lp)) return dispatch_lds<T, CH>(lp, op, in, out, s);
   fill_offsets<T, CH>(p, op.off_in, op.off_out);
   return dispatch_vec_small<T, CH, InlineOp>(p, op, in, out, s);
 }
@@ -347,9 +809,11 @@ static int run_ptr(const Plan& p, const void* ptm_dev, const void* in, void* out
                    hipStream_t s) {
   PtrOp<T, CH> op;
   memset(&op, 0, sizeof(op));
-  fill_offsets<T, CH>(p, op.off_in, op.off_out);
   op.ptm = static_cast<const T*>(ptm_dev);
   op.ld = p.din;
+  LdsPlan lp;
+  if (make_lds_plan(p, sizeof(T), lp)) return dispatch_lds<T, CH>(lp, op, in, out, s);
+  fill_offsets<T, CH>(p, op.off_in, op.off_out);
   return dispatch_vec_small<T, CH, PtrOp>(p, op, in, out, s);
 }
 
@@ -494,4 +958,130 @@ extern "C" int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
                 p.din, (long long)need);
   return dtype == QSV_F64 ? run_dev_table<double>(p, ptm_dev, workspace, in, out, s)
                           : run_dev_table<float>(p, ptm_dev, workspace, in, out, s);
+}
+
+// ------------------------------------------------------ host emulation
+// Replays the planner and the per-thread index arithmetic of the kernel
+// path the GPU call would take, on the CPU in float64, checking every
+// global and LDS access against its buffer and that every output element is
+// written exactly once.  Diagnostic only: no GPU work.
+namespace qsv {
+
+static int emulate_direct(const Plan& p, const double* ptm, const double* in, double* out) {
+  std::vector<int> written(p.n_out, 0);
+  const int vec = p.vec;
+  for (uint32_t t = 0; t < p.g.n_threads; ++t) {
+    int64_t bin, bout;
+    orbit_base(p.g, t * vec, bin, bout);
+    for (int c = 0; c < vec; ++c) {
+      std::vector<double> v(p.din);
+      for (int i = 0; i < p.din; ++i) {
+        const int64_t a = bin + p.off_in[i] + c;
+        QSV_CHECK_ARG(a >= 0 && a < p.n_in, "direct: load %lld out of bounds", (long long)a);
+        v[i] = in[a];
+      }
+      for (int o = 0; o < p.dout; ++o) {
+        double acc = 0;
+        for (int i = 0; i < p.din; ++i) acc += ptm[int64_t(o) * p.din + i] * v[i];
+        const int64_t a = bout + p.off_out[o] + c;
+        QSV_CHECK_ARG(a >= 0 && a < p.n_out, "direct: store %lld out of bounds", (long long)a);
+        out[a] = acc;
+        ++written[a];
+      }
+    }
+  }
+  for (int64_t a = 0; a < p.n_out; ++a)
+    QSV_CHECK_ARG(written[a] == 1, "direct: output %lld written %d times", (long long)a,
+                  written[a]);
+  return QSV_OK;
+}
+
+static int emulate_lds(const Plan& p, const LdsPlan& lp, int elem_bytes, const double* ptm,
+                       const double* in, double* out) {
+  const TileGeom& tg = lp.tg;
+  const int vec = lp.vec;
+  const int64_t lds_elems = int64_t(lp.lds_bytes / elem_bytes);
+  const int out_base = tg.in_place ? 0 : lds_pad(tg.runs_in * tg.L_in) + 1;
+  std::vector<int> written(p.n_out, 0);
+  QSV_CHECK_ARG(tg.L_in % vec == 0 && tg.L_out % vec == 0, "lds: run length not a multiple "
+                "of the vector width");
+  for (uint32_t tile = 0; tile < tg.n_tiles; ++tile) {
+    std::vector<double> lds(lds_elems, 0.0);
+    int64_t gin, gout;
+    tile_base(tg, tile, gin, gout);
+    for (int a = 0; a < tg.runs_in; ++a)
+      for (int e = 0; e < tg.L_in; ++e) {
+        const int64_t g = gin + tg.run_in[a] + e;
+        QSV_CHECK_ARG(g >= 0 && g < p.n_in, "lds: load %lld out of bounds", (long long)g);
+        QSV_CHECK_ARG(((gin + tg.run_in[a]) % vec) == 0, "lds: misaligned run");
+        const int l = lds_pad(a * tg.L_in + e);
+        QSV_CHECK_ARG(l >= 0 && l < lds_elems, "lds: LDS write %d out of bounds", l);
+        lds[l] = in[g];
+      }
+    QSV_CHECK_ARG(tg.orbits <= kBlock * lp.orb, "lds: %d orbits exceed %d threads x %d",
+                  tg.orbits, kBlock, lp.orb);
+    QSV_CHECK_ARG(tg.din <= lp.ch && tg.dout <= lp.ch, "lds: PTM wider than the register tile");
+    std::vector<std::vector<double>> vals(tg.orbits);
+    std::vector<int> bouts(tg.orbits, 0);
+    for (int tid = 0; tid < tg.orbits; ++tid) {
+      int bin, bout;
+      tile_orbit_base(tg, tid, bin, bout);
+      bouts[tid] = bout;
+      vals[tid].resize(tg.din);
+      for (int i = 0; i < tg.din; ++i) {
+        const int l = lds_pad(bin + tg.slot_in[i]);
+        QSV_CHECK_ARG(l >= 0 && l < lds_pad(tg.runs_in * tg.L_in) + 1,
+                      "lds: LDS read %d out of bounds", l);
+        vals[tid][i] = lds[l];
+      }
+    }
+    std::vector<int> lds_written(lds_elems, 0);
+    for (int tid = 0; tid < tg.orbits; ++tid) {
+      for (int o = 0; o < tg.dout; ++o) {
+        double acc = 0;
+        for (int i = 0; i < tg.din; ++i) acc += ptm[int64_t(o) * p.din + i] * vals[tid][i];
+        const int l = out_base + lds_pad(bouts[tid] + tg.slot_out[o]);
+        QSV_CHECK_ARG(l >= 0 && l < lds_elems, "lds: LDS result write %d out of bounds", l);
+        lds[l] = acc;
+        ++lds_written[l];
+      }
+    }
+    for (int a = 0; a < tg.runs_out; ++a)
+      for (int e = 0; e < tg.L_out; ++e) {
+        const int64_t g = gout + tg.run_out[a] + e;
+        QSV_CHECK_ARG(g >= 0 && g < p.n_out, "lds: store %lld out of bounds", (long long)g);
+        const int l = out_base + lds_pad(a * tg.L_out + e);
+        QSV_CHECK_ARG(l >= 0 && l < lds_elems, "lds: LDS read-out %d out of bounds", l);
+        QSV_CHECK_ARG(lds_written[l] == 1, "lds: LDS element %d written %d times before "
+                      "read-out", l, lds_written[l]);
+        out[g] = lds[l];
+        ++written[g];
+      }
+  }
+  for (int64_t a = 0; a < p.n_out; ++a)
+    QSV_CHECK_ARG(written[a] == 1, "lds: output %lld written %d times", (long long)a,
+                  written[a]);
+  return QSV_OK;
+}
+
+}  // namespace qsv
+
+extern "C" int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, int rank,
+                                     const int64_t* dims, int k, const int32_t* targets,
+                                     const int32_t* dims_out_t, const double* ptm_f64,
+                                     int32_t* path_out) {
+  Plan p;
+  const int rc = make_plan(dtype, rank, dims, k, targets, dims_out_t, p);
+  if (rc) return rc;
+  QSV_CHECK_ARG(in && out && ptm_f64, "null pointer argument");
+  QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24) && p.n_out <= (int64_t(1) << 24),
+                "emulation is for small vectors");
+  const int es = dtype_size(dtype);
+  LdsPlan lp;
+  if (fits_inline(p.din, p.dout) && make_lds_plan(p, es, lp)) {
+    if (path_out) *path_out = 2;
+    return emulate_lds(p, lp, es, ptm_f64, in, out);
+  }
+  if (path_out) *path_out = fits_inline(p.din, p.dout) ? 1 : 3;
+  return emulate_direct(p, ptm_f64, in, out);
 }
