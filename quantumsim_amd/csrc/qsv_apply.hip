@@ -15,6 +15,8 @@
 // across the grid and are read with scalar loads (kernel arguments or a small
 // device table); nothing is staged per block.
 #include "qsv_common.h"
+#include "qsv_geom.h"
+#include "qsv_fused.h"
 
 #include <algorithm>
 #include <stdlib.h>
@@ -24,42 +26,7 @@
 
 namespace qsv {
 
-constexpr int kMaxGroups = QSV_MAX_TARGETS + 1;
-constexpr int kBlock = 256;
 constexpr int kInline = 16;  // PTMs up to 16 x 16 travel in kernel arguments
-
-struct Geom {
-  uint32_t n_threads;  // orbits / VEC
-  int32_t ngroups;     // bystander groups, innermost first
-  int32_t din, dout;   // products of the target input / output dims
-  FastDiv gdiv[kMaxGroups];
-  int64_t gs_in[kMaxGroups];
-  int64_t gs_out[kMaxGroups];
-};
-
-template <typename T, int VEC>
-struct VT {
-  typedef T t __attribute__((ext_vector_type(VEC)));
-};
-template <typename T>
-struct VT<T, 1> {
-  typedef T t;
-};
-
-// Global accesses of the streaming kernels.  NT = nontemporal (streaming)
-// hint, used when the vector is too large to stay cache-resident between
-// consecutive gates.
-template <bool NT, typename V>
-__device__ __forceinline__ V gload(const V* p) {
-  if constexpr (NT) return __builtin_nontemporal_load(p);
-  else return *p;
-}
-template <bool NT, typename V>
-__device__ __forceinline__ void gstore(V* p, const V& v) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
-
 // --- operand sources.  p(o, i) is only called with o < dout, i < din.
 // PTM padded to CH x CH (zeros) and both offset tables, by value.
 template <typename T, int CH>
@@ -98,26 +65,6 @@ struct TableOp {
   int32_t ld;
   int32_t padded;
 };
-
-__host__ __device__ __forceinline__ void orbit_base(const Geom& g, uint32_t m, int64_t& bin,
-                                                    int64_t& bout) {
-  bin = 0;
-  bout = 0;
-#pragma unroll
-  for (int j = 0; j < kMaxGroups; ++j) {
-    if (j >= g.ngroups) break;
-    if (j == g.ngroups - 1) {
-      bin += int64_t(m) * g.gs_in[j];
-      bout += int64_t(m) * g.gs_out[j];
-    } else {
-      const uint32_t q = fdiv(m, g.gdiv[j]);
-      const uint32_t r = m - q * g.gdiv[j].d;
-      bin += int64_t(r) * g.gs_in[j];
-      bout += int64_t(r) * g.gs_out[j];
-      m = q;
-    }
-  }
-}
 
 // ---------------------------------------------------------------- small
 // din, dout <= CH: all D_in operands live in registers, outputs stream out.
@@ -854,6 +801,33 @@ static int64_t table_bytes(int dtype, int64_t din, int64_t dout, bool with_ptm) 
   return round_up(b, 256);
 }
 
+static bool use_fused(const Plan& p) {
+  return p.din == p.dout && (p.din == 64 || p.din == 256);
+}
+
+// Dense 64 x 64 / 256 x 256 PTMs on MFMA: the workspace holds off_in[D],
+// off_out[D] and the PTM in A-fragment order.
+template <typename T>
+static int run_fused(const Plan& p, const double* ptm_f64, void* workspace, const void* in,
+                     void* out, hipStream_t s) {
+  const int D = p.din;
+  const int64_t need = (2 * int64_t(D)) * 8 + int64_t(D) * D * int64_t(sizeof(T));
+  std::vector<unsigned char> host(need, 0);
+  int64_t* oin = reinterpret_cast<int64_t*>(host.data());
+  int64_t* oout = oin + D;
+  for (int i = 0; i < D; ++i) oin[i] = p.off_in[i];
+  for (int o = 0; o < D; ++o) oout[o] = p.off_out[o];
+  T* pf = reinterpret_cast<T*>(host.data() + 2 * int64_t(D) * 8);
+  for (int o = 0; o < D; ++o)
+    for (int k = 0; k < D; ++k) pf[fused_frag_index(D, o, k)] = T(ptm_f64[int64_t(o) * D + k]);
+  QSV_CHECK_HIP(hipMemcpyAsync(workspace, host.data(), need, hipMemcpyHostToDevice, s));
+  const unsigned char* wb = static_cast<const unsigned char*>(workspace);
+  const uint32_t n_orbits = uint32_t(int64_t(p.g.n_threads) * p.vec);
+  return launch_fused(sizeof(T) == 8 ? QSV_F64 : QSV_F32, D, p.g, n_orbits, in, out,
+                      wb + 2 * int64_t(D) * 8, reinterpret_cast<const int64_t*>(wb),
+                      reinterpret_cast<const int64_t*>(wb) + D, s);
+}
+
 template <typename T>
 static int run_host_table(const Plan& p, const double* ptm_f64, void* workspace,
                           const void* in, void* out, hipStream_t s) {
@@ -930,6 +904,9 @@ extern "C" int qsv_apply_ptm(int dtype, const void* in, void* out, int rank,
   QSV_CHECK_ARG(workspace && workspace_bytes >= need,
                 "a %d x %d PTM needs a workspace of %lld bytes", p.dout, p.din,
                 (long long)need);
+  if (use_fused(p))
+    return dtype == QSV_F64 ? run_fused<double>(p, ptm_f64, workspace, in, out, s)
+                            : run_fused<float>(p, ptm_f64, workspace, in, out, s);
   return dtype == QSV_F64 ? run_host_table<double>(p, ptm_f64, workspace, in, out, s)
                           : run_host_table<float>(p, ptm_f64, workspace, in, out, s);
 }
@@ -1064,6 +1041,71 @@ static int emulate_lds(const Plan& p, const LdsPlan& lp, int elem_bytes, const d
   return QSV_OK;
 }
 
+// Replays k_apply_fused: tile gather, fragment-ordered PTM reads and the
+// accumulator-to-HBM mapping of every lane.
+static int emulate_fused(const Plan& p, int elem_bytes, const double* ptm, const double* in,
+                         double* out) {
+  const int D = p.din, G = fused_cols(D), MB = D / 64, NB = G / 16, KS = D / 4;
+  const uint32_t n_orbits = uint32_t(int64_t(p.g.n_threads) * p.vec);
+  std::vector<double> pf(int64_t(D) * D);
+  for (int o = 0; o < D; ++o)
+    for (int k = 0; k < D; ++k) {
+      const int64_t f = fused_frag_index(D, o, k);
+      QSV_CHECK_ARG(f >= 0 && f < int64_t(D) * D, "fused: fragment index out of range");
+      pf[f] = ptm[int64_t(o) * D + k];
+    }
+  std::vector<int> written(p.n_out, 0);
+  const uint32_t tiles = (n_orbits + G - 1) / G;
+  std::vector<double> Bs(size_t(D) * (G + 4));
+  for (uint32_t t = 0; t < tiles; ++t) {
+    const uint32_t g0 = t * G;
+    for (int tid = 0; tid < kBlock; ++tid) {
+      const int c = tid % G;
+      const uint32_t gi = g0 + c;
+      int64_t bin = 0, bout = 0;
+      if (gi < n_orbits) orbit_base(p.g, gi, bin, bout);
+      for (int k = tid / G; k < D; k += kBlock / G) {
+        double x = 0;
+        if (gi < n_orbits) {
+          const int64_t a = bin + p.off_in[k];
+          QSV_CHECK_ARG(a >= 0 && a < p.n_in, "fused: load %lld out of bounds", (long long)a);
+          x = in[a];
+        }
+        Bs[size_t(k) * (G + 4) + c] = x;
+      }
+    }
+    for (int w = 0; w < 4; ++w)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int nb = 0; nb < NB; ++nb) {
+          const uint32_t gi = g0 + 16 * nb + (lane & 15);
+          if (gi >= n_orbits) continue;
+          int64_t bin, bout;
+          orbit_base(p.g, gi, bin, bout);
+          for (int mb = 0; mb < MB; ++mb)
+            for (int r = 0; r < 4; ++r) {
+              const int row = w * (D / 4) + 16 * mb + mfma_acc_row(elem_bytes, lane, r);
+              // the accumulator element = sum over k-steps of A(row, k) B(k, col)
+              double acc = 0;
+              for (int sk = 0; sk < KS; ++sk)
+                for (int kk = 0; kk < 4; ++kk) {
+                  const int k = 4 * sk + kk;
+                  acc += pf[fused_frag_index(D, row, k)] *
+                         Bs[size_t(k) * (G + 4) + 16 * nb + (lane & 15)];
+                }
+              const int64_t a = bout + p.off_out[row];
+              QSV_CHECK_ARG(a >= 0 && a < p.n_out, "fused: store %lld out of bounds",
+                            (long long)a);
+              out[a] = acc;
+              ++written[a];
+            }
+        }
+  }
+  for (int64_t a = 0; a < p.n_out; ++a)
+    QSV_CHECK_ARG(written[a] == 1, "fused: output %lld written %d times", (long long)a,
+                  written[a]);
+  return QSV_OK;
+}
+
 }  // namespace qsv
 
 extern "C" int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, int rank,
@@ -1081,6 +1123,10 @@ extern "C" int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, i
   if (fits_inline(p.din, p.dout) && make_lds_plan(p, es, lp)) {
     if (path_out) *path_out = 2;
     return emulate_lds(p, lp, es, ptm_f64, in, out);
+  }
+  if (!fits_inline(p.din, p.dout) && use_fused(p)) {
+    if (path_out) *path_out = 4;
+    return emulate_fused(p, es, ptm_f64, in, out);
   }
   if (path_out) *path_out = fits_inline(p.din, p.dout) ? 1 : 3;
   return emulate_direct(p, ptm_f64, in, out);

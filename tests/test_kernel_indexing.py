@@ -50,7 +50,10 @@ for n in (1, 2, 3, 5, 7):
             GEOMS.append(([4] * n, t))
 GEOMS += [([2] * 6, (5, 2, 0)), ([2] * 6, (0, 1, 2, 3)), ([2] * 7, (6, 5, 4, 3)),
           ([9, 4, 4, 9], (3,)), ([9, 4, 4, 9], (0, 3)), ([4, 9, 4, 3], (2, 3)),
-          ([3, 4, 2, 4, 4], (4, 3)), ([1, 4, 4, 1], (3, 2)), ([4, 4, 4, 4, 4, 4], (4,))]
+          ([3, 4, 2, 4, 4], (4, 3)), ([1, 4, 4, 1], (3, 2)), ([4, 4, 4, 4, 4, 4], (4,)),
+          # dense 64 x 64 and 256 x 256 PTMs (MFMA path)
+          ([4] * 5, (0, 1, 2)), ([4] * 5, (4, 2, 0)), ([4] * 6, (1, 2, 3, 4)),
+          ([4] * 4, (3, 0, 1, 2)), ([2] * 8, (7, 0, 3, 1, 5, 2)), ([4, 4, 4, 4, 3], (0, 1, 2))]
 
 
 @pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
@@ -58,7 +61,10 @@ GEOMS += [([2] * 6, (5, 2, 0)), ([2] * 6, (0, 1, 2, 3)), ([2] * 7, (6, 5, 4, 3))
 def test_emulated_kernels_match_oracle(dims, targets, dtype, rng):
     pv = rng.standard_normal(dims)
     din = [dims[t] for t in targets]
-    for dout in (din, [max(1, d // 2) for d in din], [min(d + 1, 16) for d in din]):
+    douts = [din]
+    if np.prod(din) <= 16:
+        douts += [[max(1, d // 2) for d in din], [min(d + 1, 16) for d in din]]
+    for dout in douts:
         if np.prod(dout) > 16 and np.prod(din) <= 16:
             continue
         ptm = rng.standard_normal(list(dout) + din)
@@ -79,3 +85,5 @@ def test_paths_taken():
     assert emulate(nat.QSV_F64, pv, ptm, (2, 3))[1] == 1
     big = rng.standard_normal([9] * 4)
     assert emulate(nat.QSV_F64, big, rng.standard_normal([9] * 4), (1, 2))[1] == 3
+    assert emulate(nat.QSV_F64, pv, rng.standard_normal([4] * 6), (1, 2, 3))[1] == 4
+    assert emulate(nat.QSV_F32, pv, rng.standard_normal([4] * 8), (0, 7, 3, 4))[1] == 4
