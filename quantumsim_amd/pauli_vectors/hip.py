@@ -347,6 +347,38 @@ class PauliVectorHIP(PauliVectorBase):
             return d
         return d.reshape(-1).cpu().numpy()
 
+    def multitake(self, *indices, get_data=True):
+        """``to_pv()[np.ix_(*indices)]``, one index list per qudit, gathered
+        on the device bit-exactly (the reference's multitake kernel,
+        quantumsim/pauli_vectors/primitives.cu:92-130).  Returns float64 (a
+        device tensor with ``get_data=False``)."""
+        if len(indices) != len(self._shape):
+            raise ValueError('multitake needs one index list per qudit ({}), got {}'
+                             .format(len(self._shape), len(indices)))
+        lists = []
+        for q, idx in enumerate(indices):
+            idx = [int(i) for i in np.asarray(idx).reshape(-1)]
+            for i in idx:
+                if i < 0 or i >= self._shape[q]:
+                    raise ValueError('index {} out of range for qudit {} of dimension {}'
+                                     .format(i, q, self._shape[q]))
+            lists.append(idx)
+        flat, offsets = [], []
+        for idx in lists:
+            offsets.append(len(flat))
+            flat.extend(idx)
+        out_dims = [len(idx) for idx in lists]
+        n_out = math.prod(out_dims)
+        out = torch.empty(max(n_out, 1), dtype=torch.float64, device=self._device)
+        table = torch.tensor(flat if flat else [0], dtype=torch.int32, device=self._device)
+        rc = nat.lib().qsv_multitake(self._code(), nat.QSV_F64, self._data.data_ptr(),
+                                     out.data_ptr(), len(self._shape), nat.i64(self._shape),
+                                     nat.i64(out_dims), table.data_ptr(), nat.i32(offsets),
+                                     _stream(self._device))
+        nat.check(rc, 'multitake')
+        res = out[:n_out].view(tuple(out_dims))
+        return res.cpu().numpy() if get_data else res
+
     def _axis_sum(self, t, n_outer, d, n_inner):
         L = nat.lib()
         ws_bytes = nat.check(L.qsv_axis_sum_workspace_bytes(n_outer, d, n_inner), 'sum')

@@ -19,7 +19,8 @@ from .algebra.algebra import kraus_to_ptm
 from .bases import general
 from .operations import qubits as gates
 
-__all__ = ['haar_unitary', 'noisy_layers', 'layer_slices', 'damping_ptm', 'circuit_bytes']
+__all__ = ['haar_unitary', 'noisy_layers', 'layer_slices', 'damping_ptm', 'circuit_bytes',
+           'random_kraus', 'mixed_qudit_program']
 
 _B1 = (general(2),)
 GATE_TIME = 20e-9
@@ -83,3 +84,45 @@ def circuit_bytes(circuit, n_qubits, elem_bytes, dim=4):
     negligible and ignored)."""
     n = dim ** n_qubits
     return len(circuit) * 2 * n * elem_bytes
+
+
+def random_kraus(dim, n_kraus, rng):
+    """n_kraus Kraus operators of a random CPTP map on a dim-level system:
+    a Haar-like random isometry dim -> n_kraus * dim (QR of a complex
+    Gaussian matrix) cut into n_kraus blocks, so sum_k K_k^dag K_k = 1."""
+    z = rng.standard_normal((n_kraus * dim, dim)) + 1j * rng.standard_normal((n_kraus * dim, dim))
+    q, r = np.linalg.qr(z)
+    q = q * (np.diagonal(r) / np.abs(np.diagonal(r)))
+    return q.reshape(n_kraus, dim, dim)
+
+
+def mixed_qudit_program(dims, n_steps, seed):
+    """Mixed qubit/qutrit workload (10 qudits, 6 of them qutrits, in the
+    benchmark configuration).  Qutrits live in the Gell-Mann basis, qubits
+    in the full qubit basis until measured, after which they are reduced to
+    their diagonal subbasis.  Each step is a dict with
+      pair    : the two qudits of a random 2-qudit CPTP map (4 Kraus ops)
+      kraus   : its Kraus operators on the pair's Hilbert space
+      measure : a qubit whose populations are read out, then whose basis is
+                reduced to subbasis([0, 1]) (a 0/1 selection PTM)
+      keep    : three qudits for partial_trace
+      take    : per-qudit index lists (for those three qudits) for multitake
+    The PTMs depend on the bases at each point and are built by the runner
+    with kraus_to_ptm."""
+    rng = np.random.default_rng(seed)
+    n = len(dims)
+    qubits = [q for q in range(n) if dims[q] == 2]
+    steps = []
+    for _ in range(n_steps):
+        pair = tuple(int(x) for x in rng.choice(n, 2, replace=False))
+        D = dims[pair[0]] * dims[pair[1]]
+        keep = tuple(sorted(int(x) for x in rng.choice(n, 3, replace=False)))
+        steps.append({
+            'pair': pair,
+            'kraus': random_kraus(D, 4, rng),
+            'measure': int(rng.choice(qubits)),
+            'keep': keep,
+            'take': [sorted(int(x) for x in rng.choice(dims[q] ** 2, 2, replace=False))
+                     for q in keep],
+        })
+    return steps
