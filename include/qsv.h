@@ -73,6 +73,38 @@ int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
                       const int32_t* dims_out_t, const void* ptm_dev,
                       void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Batched form of qsv_apply_ptm_dev for PTMs up to 16 x 16: `batch` states
+ * of identical shape, stored back to back (in: batch x N_in, out: batch x
+ * N_out), each contracted with its own device PTM
+ * ptm_dev + b * ptm_batch_stride (elements of the state's type; a stride of
+ * 0 applies one PTM to every state).  One launch, no host-device copies:
+ * safe under HIP graph capture.  No counterpart in the reference (its
+ * backend holds one state per object); used by the circuit batcher and the
+ * differentiable path, which run many circuits or parameter sets at once. */
+int qsv_apply_ptm_batched(int dtype, const void* in, void* out, int batch, int rank,
+                          const int64_t* dims, int k, const int32_t* targets,
+                          const int32_t* dims_out_t, const void* ptm_dev,
+                          int64_t ptm_batch_stride, void* stream);
+
+/* Bytes of device workspace qsv_ptm_grad needs. */
+int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64_t* dims, int k,
+                                     const int32_t* targets, const int32_t* dims_out_t);
+
+/* Gradient of a PTM application with respect to the PTM, for `batch` states
+ * back to back (the layout of qsv_apply_ptm_batched), PTMs up to 16 x 16:
+ *   dptm[b, o, i] (+)= sum over orbits of ybar[b, .., o, ..] * x[b, .., i, ..]
+ * x: the states the PTM was applied to; ybar: the loss gradient with
+ * respect to the results.  Accumulated in float64 in a fixed order
+ * (deterministic), written (accumulate = 0) or added (accumulate = 1) to
+ * dptm in the state's element type.  The gradient with respect to x is
+ * qsv_apply_ptm_batched with the transposed PTMs.  No counterpart in the
+ * reference (which has no differentiation); it is the adjoint used by the
+ * differentiable path. */
+int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batch, int rank,
+                 const int64_t* dims, int k, const int32_t* targets, const int32_t* dims_out_t,
+                 void* dptm, int accumulate, void* workspace, int64_t workspace_bytes,
+                 void* stream);
+
 /* out = in[np.ix_(idx_0, ..., idx_{rank-1})] with an index of -1 producing
  * 0.  The index lists live in device memory as int32: idx_dev[offsets[r] +
  * j] is the j-th index of axis r, for j < out_dims[r]; offsets (host) has

@@ -41,6 +41,14 @@ _SIGNATURES = {
     'qsv_apply_ptm_dev': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, _c_i64p,
                                          ctypes.c_int, _c_i32p, _c_i32p, _vp, _vp,
                                          ctypes.c_int64, _vp]),
+    'qsv_apply_ptm_batched': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                             _c_i64p, ctypes.c_int, _c_i32p, _c_i32p, _vp,
+                                             ctypes.c_int64, _vp]),
+    'qsv_ptm_grad_workspace_bytes': (ctypes.c_int64, [ctypes.c_int, ctypes.c_int, _c_i64p,
+                                                      ctypes.c_int, _c_i32p, _c_i32p]),
+    'qsv_ptm_grad': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_i64p,
+                                    ctypes.c_int, _c_i32p, _c_i32p, _vp, ctypes.c_int, _vp,
+                                    ctypes.c_int64, _vp]),
     'qsv_multitake': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                      _c_i64p, _c_i64p, _vp, _c_i32p, _vp]),
     'qsv_axis_sum_workspace_bytes': (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int,

@@ -1,0 +1,344 @@
+"""Differentiable, batched Pauli-vector circuits.
+
+States are float tensors of shape [B, N] on the GPU (B circuits or
+parameter sets of one qudit shape, N = prod dim_pauli, C order, qudit 0
+most significant).  Gates carry per-state PTMs [B, D_out, D_in] (or one
+shared PTM [1, D_out, D_in]) built from parameters with torch ops, so
+autograd reaches the parameters through them.
+
+* ``apply`` contracts a PTM with the states in one libqsv launch
+  (qsv_apply_ptm_batched); its backward is the hand-written adjoint:
+  x_bar = P^T y_bar (same kernel, transposed PTM) and
+  P_bar = sum over orbits y_bar x^T (qsv_ptm_grad, float64 accumulation).
+* ``run_circuit`` applies a gate list and keeps a checkpoint every
+  ``checkpoint_every`` gates; backward recomputes each segment from its
+  checkpoint and runs the adjoint through it, so memory is
+  (#gates / k + k) states instead of #gates states.
+* ``diagonal`` gathers the populations (qsv_multitake, float64 out); its
+  backward scatters the incoming gradient onto the gathered entries.
+
+No counterpart in the reference, which has no differentiation; the PTM
+constructors follow quantumsim/operations/qubits.py:38-90,182-220 and
+quantumsim/algebra/algebra.py:11-30.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .bases import general
+
+__all__ = ['apply', 'run_circuit', 'diagonal', 'Gate', 'qubit_basis_tensor', 'kraus_ptm',
+           'rx_ptm', 'rz_ptm', 'damping_ptm', 'cz_ptm', 'kron_ptm', 'chain_ptm']
+
+
+def _stream():
+    s = torch.cuda.current_stream().cuda_stream
+    return s if s else None
+
+
+class _Workspace:
+    _bufs = {}
+
+    @classmethod
+    def get(cls, device, nbytes):
+        t = cls._bufs.get(device)
+        if t is None or t.numel() < nbytes:
+            t = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+            cls._bufs[device] = t
+        return t
+
+
+def _apply_raw(x, ptm, shape, targets, dims_out):
+    """y[b] = ptm[b or 0] applied to x[b] on `targets` (no autograd)."""
+    B = x.shape[0]
+    n_out = math.prod(shape[q] if q not in targets else dims_out[targets.index(q)]
+                      for q in range(len(shape)))
+    y = torch.empty((B, n_out), dtype=x.dtype, device=x.device)
+    ptm = ptm.to(dtype=x.dtype).contiguous()
+    pstride = 0 if ptm.shape[0] == 1 else ptm.shape[1] * ptm.shape[2]
+    rc = nat.lib().qsv_apply_ptm_batched(
+        nat.dtype_code(x.dtype), x.data_ptr(), y.data_ptr(), B, len(shape), nat.i64(shape),
+        len(targets), nat.i32(targets), nat.i32(dims_out), ptm.data_ptr(), pstride, _stream())
+    nat.check(rc, 'apply')
+    return y
+
+
+def _ptm_grad_raw(x, ybar, shape, targets, dims_out):
+    B = x.shape[0]
+    L = nat.lib()
+    k = len(targets)
+    din = math.prod(shape[q] for q in targets)
+    dout = math.prod(dims_out)
+    ws = nat.check(L.qsv_ptm_grad_workspace_bytes(B, len(shape), nat.i64(shape), k,
+                                                  nat.i32(targets), nat.i32(dims_out)),
+                   'ptm_grad')
+    buf = _Workspace.get(x.device, ws)
+    out = torch.empty((B, dout, din), dtype=x.dtype, device=x.device)
+    rc = L.qsv_ptm_grad(nat.dtype_code(x.dtype), x.data_ptr(), ybar.data_ptr(), B, len(shape),
+                        nat.i64(shape), k, nat.i32(targets), nat.i32(dims_out),
+                        out.data_ptr(), 0, buf.data_ptr(), ws, _stream())
+    nat.check(rc, 'ptm_grad')
+    return out
+
+
+class _Apply(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, ptm, shape, targets, dims_out):
+        ctx.save_for_backward(x, ptm)
+        ctx.geo = (shape, targets, dims_out)
+        return _apply_raw(x.contiguous(), ptm, shape, targets, dims_out)
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, ptm = ctx.saved_tensors
+        shape, targets, dims_out = ctx.geo
+        gy = gy.contiguous()
+        gx = gp = None
+        out_shape = tuple(shape[q] if q not in targets else dims_out[targets.index(q)]
+                          for q in range(len(shape)))
+        if ctx.needs_input_grad[0]:
+            gx = _apply_raw(gy, ptm.transpose(1, 2), out_shape, targets,
+                            [shape[q] for q in targets])
+        if ctx.needs_input_grad[1]:
+            gp = _ptm_grad_raw(x.contiguous(), gy, shape, targets, dims_out)
+            if ptm.shape[0] == 1 and x.shape[0] != 1:
+                gp = gp.sum(0, keepdim=True)
+            gp = gp.to(ptm.dtype)
+        return gx, gp, None, None, None
+
+
+def apply(x, ptm, shape, targets, dims_out=None):
+    """Differentiable batched PTM application.  x: [B, N]; ptm: [B or 1,
+    D_out, D_in] (PTM axis order (out..., in...) flattened)."""
+    targets = tuple(int(t) for t in targets)
+    if dims_out is None:
+        dims_out = tuple(shape[q] for q in targets)
+    if ptm.dim() == 2:
+        ptm = ptm.unsqueeze(0)
+    return _Apply.apply(x, ptm, tuple(shape), targets, tuple(dims_out))
+
+
+class Gate:
+    """One gate of a circuit: square PTMs [B or 1, D, D] on ``targets``."""
+    __slots__ = ('ptm', 'targets')
+
+    def __init__(self, ptm, targets):
+        self.ptm = ptm if ptm.dim() == 3 else ptm.unsqueeze(0)
+        self.targets = tuple(int(t) for t in targets)
+
+
+class _Circuit(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x0, meta, *ptms):
+        shape, targets_list, k = meta
+        ckpts = [x0]
+        x = x0
+        with torch.no_grad():
+            for j, (ptm, tg) in enumerate(zip(ptms, targets_list)):
+                x = _apply_raw(x, ptm, shape, tg, [shape[q] for q in tg])
+                if (j + 1) % k == 0 and j + 1 < len(ptms):
+                    ckpts.append(x)
+        ctx.meta = meta
+        ctx.n_ckpt = len(ckpts)
+        ctx.save_for_backward(*ckpts, *ptms)
+        return x
+
+    @staticmethod
+    def backward(ctx, gy):
+        shape, targets_list, k = ctx.meta
+        saved = ctx.saved_tensors
+        ckpts, ptms = saved[:ctx.n_ckpt], saved[ctx.n_ckpt:]
+        G = len(ptms)
+        grads = [None] * G
+        gy = gy.contiguous()
+        for seg in reversed(range(ctx.n_ckpt)):
+            lo, hi = seg * k, min(G, (seg + 1) * k)
+            xs = [ckpts[seg]]
+            for j in range(lo, hi - 1):
+                tg = targets_list[j]
+                xs.append(_apply_raw(xs[-1], ptms[j], shape, tg, [shape[q] for q in tg]))
+            for j in reversed(range(lo, hi)):
+                tg = targets_list[j]
+                d = [shape[q] for q in tg]
+                if ctx.needs_input_grad[2 + j]:
+                    g = _ptm_grad_raw(xs[j - lo], gy, shape, tg, d)
+                    if ptms[j].shape[0] == 1 and gy.shape[0] != 1:
+                        g = g.sum(0, keepdim=True)
+                    grads[j] = g.to(ptms[j].dtype)
+                gy = _apply_raw(gy, ptms[j].transpose(1, 2), shape, tg, d)
+        return (gy, None, *grads)
+
+
+def run_circuit(x0, gates, shape, checkpoint_every=8):
+    """Apply ``gates`` (list of Gate, square PTMs) to states x0 [B, N];
+    differentiable in x0 and every gate PTM, with checkpointing."""
+    ptms = [g.ptm for g in gates]
+    meta = (tuple(shape), [g.targets for g in gates], max(1, int(checkpoint_every)))
+    return _Circuit.apply(x0, meta, *ptms)
+
+
+_DIAG_TABLES = {}
+
+
+def _scatter_index(shape, take, device):
+    """(source flat index, gathered position) of every gathered entry that
+    exists in the basis; device tensors made once, outside graph capture."""
+    flat = np.zeros([len(t) for t in take], dtype=np.int64)
+    valid = np.ones_like(flat, dtype=bool)
+    stride = 1
+    for q in reversed(range(len(shape))):
+        t = np.asarray(take[q])
+        sl = [None] * len(shape)
+        sl[q] = slice(None)
+        flat = flat + (np.where(t < 0, 0, t) * stride)[tuple(sl)]
+        valid = valid & (t >= 0)[tuple(sl)]
+        stride *= shape[q]
+    v = valid.reshape(-1)
+    return (torch.from_numpy(flat.reshape(-1)[v]).to(device),
+            torch.from_numpy(np.flatnonzero(v)).to(device))
+
+
+class _Diagonal(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, shape, take, offsets, take_dims, idx_dev, scatter):
+        B = x.shape[0]
+        n = math.prod(take_dims)
+        out = torch.empty((B, n), dtype=torch.float64, device=x.device)
+        rank = len(shape) + 1
+        in_dims = [B] + list(shape)
+        out_dims = [B] + list(take_dims)
+        rc = nat.lib().qsv_multitake(nat.dtype_code(x.dtype), nat.QSV_F64, x.data_ptr(),
+                                     out.data_ptr(), rank, nat.i64(in_dims), nat.i64(out_dims),
+                                     idx_dev.data_ptr(), nat.i32(offsets), _stream())
+        nat.check(rc, 'diagonal')
+        ctx.geo = (shape, x.dtype, B, scatter)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        shape, dtype, B, (src, pos) = ctx.geo
+        gx = torch.zeros((B, math.prod(shape)), dtype=dtype, device=g.device)
+        gx.index_add_(1, src, g.index_select(1, pos).to(dtype))
+        return gx, None, None, None, None, None, None
+
+
+def diagonal(x, bases):
+    """Populations <j|rho_b|j> of every state, float64 [B, prod d_hilbert].
+    Only bases whose populations are a gather (general(d) and its subbases)
+    are supported here."""
+    take = []
+    for b in bases:
+        c = np.real(np.asarray(b.computational_basis_vectors, dtype=complex))
+        row = []
+        for r in c:
+            nz = np.flatnonzero(r)
+            if nz.size == 0:
+                row.append(-1)
+            elif nz.size == 1 and r[nz[0]] == 1.0:
+                row.append(int(nz[0]))
+            else:
+                raise NotImplementedError('differentiable diagonal needs gather bases')
+        take.append(row)
+    shape = tuple(b.dim_pauli for b in bases)
+    B = x.shape[0]
+    key = (tuple(tuple(t) for t in take), B, x.device)
+    hit = _DIAG_TABLES.get(key)
+    if hit is None:
+        # axis 0 of the gather is the batch axis (index list 0..B-1)
+        offsets, flat_all = [0], list(range(B))
+        for t in take:
+            offsets.append(len(flat_all))
+            flat_all.extend(t)
+        idx_dev = torch.tensor(flat_all, dtype=torch.int32, device=x.device)
+        hit = (offsets, idx_dev, _scatter_index(shape, take, x.device))
+        _DIAG_TABLES[key] = hit
+    offsets, idx_dev, scatter = hit
+    return _Diagonal.apply(x.contiguous(), shape, take, offsets, [len(t) for t in take],
+                           idx_dev, scatter)
+
+
+# ----------------------------------------------------------- PTM builders
+_BASIS_CACHE = {}
+
+
+def qubit_basis_tensor(device, dtype=torch.complex128):
+    """general(2) basis matrices as a [4, 2, 2] complex tensor."""
+    key = (device, dtype)
+    t = _BASIS_CACHE.get(key)
+    if t is None:
+        t = torch.tensor(np.asarray(general(2).vectors), dtype=dtype, device=device)
+        _BASIS_CACHE[key] = t
+    return t
+
+
+def kraus_ptm(kraus, basis):
+    """PTM[n, x, y] = sum_k Re tr(P_x K P_y K^dag) for batched Kraus sets
+    kraus [n, K, d, d] in the basis [X, d, d] (quantumsim/algebra/algebra.py:11-30)."""
+    kp = torch.einsum('nkab,ybc,nkdc->nyad', kraus, basis, kraus.conj())
+    return torch.einsum('xda,nyad->nxy', basis, kp).real
+
+
+def rx_ptm(theta):
+    """RX(theta) = [[c, -is], [-is, c]] (quantumsim/operations/qubits.py:38-53)."""
+    c = torch.cos(theta / 2).to(torch.complex128)
+    s = torch.sin(theta / 2).to(torch.complex128)
+    k = torch.stack([torch.stack([c, -1j * s], -1), torch.stack([-1j * s, c], -1)], -2)
+    return kraus_ptm(k.unsqueeze(-3), qubit_basis_tensor(theta.device))
+
+
+def rz_ptm(theta):
+    """RZ(theta) = diag(e^{-i theta/2}, e^{i theta/2}) (qubits.py:74-90)."""
+    e = torch.exp(-0.5j * theta.to(torch.complex128))
+    z = torch.zeros_like(e)
+    k = torch.stack([torch.stack([e, z], -1), torch.stack([z, e.conj()], -1)], -2)
+    return kraus_ptm(k.unsqueeze(-3), qubit_basis_tensor(theta.device))
+
+
+def damping_ptm(t1, t2, t):
+    """Amplitude then phase damping for a wait t under T1, T2
+    (gamma = 1 - e^{-t/T1}, 1/T_phi = 1/T2 - 1/(2 T1),
+    lambda = 1 - e^{-2t/T_phi}); Kraus sets of qubits.py:182-214."""
+    g = 1 - torch.exp(-t / t1)
+    lam = 1 - torch.exp(-2 * t * (1 / t2 - 0.5 / t1))
+    one = torch.ones_like(g)
+    zero = torch.zeros_like(g)
+    cd = lambda m: m.to(torch.complex128)
+    ka = torch.stack([
+        torch.stack([torch.stack([one, zero], -1), torch.stack([zero, torch.sqrt(1 - g)], -1)], -2),
+        torch.stack([torch.stack([zero, torch.sqrt(g)], -1), torch.stack([zero, zero], -1)], -2),
+    ], -3)
+    kp = torch.stack([
+        torch.stack([torch.stack([one, zero], -1), torch.stack([zero, torch.sqrt(1 - lam)], -1)], -2),
+        torch.stack([torch.stack([zero, zero], -1), torch.stack([zero, torch.sqrt(lam)], -1)], -2),
+    ], -3)
+    basis = qubit_basis_tensor(t1.device)
+    return kraus_ptm(cd(kp), basis) @ kraus_ptm(cd(ka), basis)
+
+
+def cz_ptm(device):
+    """CZ = cphase(pi) as a [1, 16, 16] PTM (qubits.py:109-123)."""
+    key = ('cz', device)
+    t = _BASIS_CACHE.get(key)
+    if t is None:
+        from .operations.qubits import cphase
+        t = torch.tensor(cphase(np.pi).ptm.reshape(16, 16), dtype=torch.float64,
+                         device=device).unsqueeze(0)
+        _BASIS_CACHE[key] = t
+    return t
+
+
+def kron_ptm(a, b):
+    """Two-qubit PTM of independent one-qubit PTMs a [n, 4, 4] (first
+    target) and b [n, 4, 4] (second target), PTM index order (out0 out1,
+    in0 in1)."""
+    return torch.einsum('nac,nbd->nabcd', a, b).reshape(a.shape[0], 16, 16)
+
+
+def chain_ptm(*ptms):
+    """PTM of applying ptms[0], then ptms[1], ... (batched matrix product)."""
+    out = ptms[0]
+    for p in ptms[1:]:
+        out = p @ out
+    return out

@@ -1,0 +1,161 @@
+"""Many noisy circuits of different widths and depths, simulated together.
+
+Circuits are grouped by qubit count; inside a group the states form one
+[B, 4^n] tensor ordered by decreasing depth, so the circuits still running
+at any layer are a prefix of the batch.  Every fused gate of a layer is one
+qsv_apply_ptm_batched launch over that prefix (per-circuit PTMs built on the
+device from the circuit parameters); circuits that have finished are
+retired by narrowing the prefix, without copies.  Forward, the float64
+population readout and the adjoint (checkpointed recompute + qsv_ptm_grad)
+are all device work with no host synchronisation, so the whole step can be
+captured once in a HIP graph and replayed (``CircuitBatch.capture``).
+
+The reference simulates one circuit per PauliVector object; this module has
+no counterpart there.  Its per-gate arithmetic is the reference's
+apply_ptm (quantumsim/pauli_vectors/numpy.py:50-64).
+"""
+import math
+
+import numpy as np
+import torch
+
+from .autograd import run_circuit, diagonal
+from .bases import general
+from .workloads import rotation_ptms, damping_ptms, fused_layer_gates
+
+__all__ = ['CircuitBatch', 'plain_forward_backward']
+
+
+class _Group:
+    def __init__(self, n, members, circuits, device, param_dtype):
+        self.n = n
+        self.members = members                  # indices into the circuit list
+        L = [circuits[i]['layers'] for i in members]
+        self.layers = L                         # decreasing
+        Lmax = L[0]
+        ang = np.zeros((len(members), Lmax, n, 2))
+        t1 = np.zeros((len(members), n))
+        t2 = np.zeros((len(members), n))
+        for b, i in enumerate(members):
+            c = circuits[i]
+            ang[b, :c['layers']] = c['angles']
+            t1[b], t2[b] = c['t1'], c['t2']
+        mk = lambda a: torch.tensor(a, dtype=param_dtype, device=device)
+        self.angles, self.t1, self.t2 = mk(ang), mk(t1), mk(t2)
+
+
+class CircuitBatch:
+    """Batched simulation of ``circuits`` (see workloads.random_circuit_batch).
+
+    Parameters are the leaf tensors ``group.angles`` [B, L, n, 2] and
+    ``group.t1``/``group.t2`` [B, n] (microseconds) of each group; the step
+    returns float64 populations per circuit and, with a loss, their
+    gradients with respect to every parameter."""
+
+    def __init__(self, circuits, dtype=torch.float32, device=None, checkpoint_every=16,
+                 requires_grad=True):
+        self.device = device or torch.device('cuda', torch.cuda.current_device())
+        self.dtype = dtype
+        self.k = checkpoint_every
+        self.circuits = circuits
+        # one stream (default priority) for parameters, eager steps and the
+        # captured graph, so autograd never crosses streams
+        self.stream = torch.cuda.Stream(device=self.device)
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+        by_n = {}
+        for i, c in enumerate(circuits):
+            by_n.setdefault(c['n'], []).append(i)
+        self.groups = []
+        with torch.cuda.stream(self.stream):
+            for n in sorted(by_n):
+                members = sorted(by_n[n], key=lambda i: -circuits[i]['layers'])
+                g = _Group(n, members, circuits, self.device, torch.float64)
+                for t in (g.angles, g.t1, g.t2):
+                    t.requires_grad_(requires_grad)
+                self.groups.append(g)
+        self.graph = None
+
+    def parameters(self):
+        return [t for g in self.groups for t in (g.angles, g.t1, g.t2)]
+
+    def _group_populations(self, g):
+        n, B = g.n, len(g.members)
+        N = 4 ** n
+        rot = rotation_ptms(g.angles)            # [B, L, n, 4, 4]
+        damp = damping_ptms(g.t1, g.t2)          # [B, n, 4, 4]
+        x = torch.zeros((B, N), dtype=self.dtype, device=self.device)
+        x[:, 0] = 1
+        finals = [None] * B
+        layer = 0
+        active = B
+        while active > 0:
+            # layers while the active prefix is unchanged
+            end = g.layers[active - 1]
+            gates = []
+            for l in range(layer, end):
+                gates += fused_layer_gates(n, l, rot[:active, l], damp[:active])
+            x = run_circuit(x[:active], gates, (4,) * n, self.k)
+            layer = end
+            nxt = active
+            while nxt > 0 and g.layers[nxt - 1] <= layer:
+                nxt -= 1
+            for b in range(nxt, active):
+                finals[b] = x[b:b + 1]
+            active = nxt
+            if active:
+                x = x[:active]
+        states = torch.cat(finals, 0)
+        return diagonal(states, [general(2)] * n)
+
+    def populations(self):
+        """float64 populations per group ([B_n, 2^n] tensors, depth-sorted)."""
+        return [self._group_populations(g) for g in self.groups]
+
+    def _step(self, weights):
+        pops = self.populations()
+        loss = sum((p * w).sum() for p, w in zip(pops, weights))
+        grads = torch.autograd.grad(loss, self.parameters())
+        return loss, pops, grads
+
+    def step(self, weights):
+        """Forward + adjoint of loss = sum over circuits of <w_c, p_c>;
+        returns (loss, populations, gradients of every parameter)."""
+        cur = torch.cuda.current_stream(self.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            out = self._step(weights)
+        cur.wait_stream(self.stream)
+        return out
+
+    def random_weights(self, seed):
+        g = np.random.default_rng(seed)
+        return [torch.tensor(g.standard_normal((len(gr.members), 2 ** gr.n)),
+                             dtype=torch.float64, device=self.device) for gr in self.groups]
+
+    def capture(self, weights):
+        """Record one forward + adjoint step in a HIP graph (after an eager
+        warm-up step that sizes every buffer); replay() re-runs it."""
+        self.step(weights)                       # warm-up: allocations, tables
+        torch.cuda.synchronize(self.device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            self.g_loss, self.g_pops, self.g_grads = self._step(weights)
+        torch.cuda.synchronize(self.device)
+        return self.graph
+
+    def replay(self):
+        cur = torch.cuda.current_stream(self.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            self.graph.replay()
+        cur.wait_stream(self.stream)
+        return self.g_loss, self.g_pops, self.g_grads
+
+
+def plain_forward_backward(circuit, weight, dtype=torch.float32, device=None,
+                           checkpoint_every=16):
+    """One circuit on its own (batch of 1, eager launches): the per-circuit
+    baseline the batched engine is compared with."""
+    device = device or torch.device('cuda', torch.cuda.current_device())
+    b = CircuitBatch([circuit], dtype=dtype, device=device, checkpoint_every=checkpoint_every)
+    return b.step([weight])

@@ -35,7 +35,7 @@ struct InlineOp {
   int64_t off_in[CH];
   int64_t off_out[CH];
   static constexpr bool kPadded = true;
-  __device__ __forceinline__ T p(int o, int i) const { return ptm[o * CH + i]; }
+  __device__ __forceinline__ T p(int o, int i, uint32_t) const { return ptm[o * CH + i]; }
   __device__ __forceinline__ int64_t oin(int i) const { return off_in[i]; }
   __device__ __forceinline__ int64_t oout(int o) const { return off_out[o]; }
 };
@@ -46,9 +46,12 @@ struct PtrOp {
   int64_t off_in[CH];
   int64_t off_out[CH];
   const T* __restrict__ ptm;
+  int64_t pstride;  // PTM elements between batch entries (0: one shared PTM)
   int32_t ld;
   static constexpr bool kPadded = false;
-  __device__ __forceinline__ T p(int o, int i) const { return ptm[o * ld + i]; }
+  __device__ __forceinline__ T p(int o, int i, uint32_t b) const {
+    return ptm[int64_t(b) * pstride + o * ld + i];
+  }
   __device__ __forceinline__ int64_t oin(int i) const { return off_in[i]; }
   __device__ __forceinline__ int64_t oout(int o) const { return off_out[o]; }
 };
@@ -74,6 +77,9 @@ __global__ void __launch_bounds__(kBlock)
   typedef typename VT<T, VEC>::t V;
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= g.n_threads) return;
+  const uint32_t b = blockIdx.y;
+  in += int64_t(b) * g.bs_in;
+  out += int64_t(b) * g.bs_out;
   int64_t bin, bout;
   orbit_base(g, t * VEC, bin, bout);
 
@@ -91,7 +97,7 @@ __global__ void __launch_bounds__(kBlock)
       V acc = V(0);
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        if (Op::kPadded || i < g.din) acc += op.p(o, i) * v[i];
+        if (Op::kPadded || i < g.din) acc += op.p(o, i, b) * v[i];
       }
       gstore<NT>(reinterpret_cast<V*>(out + bout + op.oout(o)), acc);
     }
@@ -170,6 +176,7 @@ struct TileGeom {
   int32_t din, dout;
   int32_t slot_in[16], slot_out[16];  // LDS offsets of the PTM indices
   int32_t in_place;
+  int64_t bs_in, bs_out;  // element stride between batch entries (grid.y)
 };
 
 __host__ __device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
@@ -226,6 +233,9 @@ __global__ void __launch_bounds__(kBlock)
   T* lin = reinterpret_cast<T*>(qsv_smem);
   T* lout = tg.in_place ? lin : lin + lds_pad(tg.runs_in * tg.L_in) + 1;
   const int tid = threadIdx.x;
+  const uint32_t b = blockIdx.y;
+  in += int64_t(b) * tg.bs_in;
+  out += int64_t(b) * tg.bs_out;
 
   int64_t gin, gout;
   tile_base(tg, blockIdx.x, gin, gout);
@@ -271,7 +281,7 @@ __global__ void __launch_bounds__(kBlock)
           T acc = T(0);
 #pragma unroll
           for (int i = 0; i < CH; ++i) {
-            if (Op::kPadded || i < tg.din) acc += op.p(o, i) * v[r][i];
+            if (Op::kPadded || i < tg.din) acc += op.p(o, i, b) * v[r][i];
           }
           lout[lds_pad(bout[r] + tg.slot_out[o])] = acc;
         }
@@ -310,6 +320,7 @@ static int64_t nt_threshold_bytes() {
 
 struct Plan {
   Geom g;
+  uint32_t batch;  // states processed by one launch (grid.y)
   int64_t n_in, n_out;
   int vec;
   bool nt;  // streaming (nontemporal) global accesses
@@ -434,6 +445,9 @@ static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int3
     ++ng;
   }
   g.ngroups = ng;
+  g.bs_in = p.n_in;
+  g.bs_out = p.n_out;
+  p.batch = 1;
   // vector width along the innermost group when it is unit-stride: 16-byte
   // accesses.  All other offsets are multiples of that group's size, so the
   // vector accesses stay aligned.
@@ -465,6 +479,7 @@ static int64_t lds_run_threshold() {
 
 struct LdsPlan {
   TileGeom tg;
+  uint32_t batch;
   int vec;
   bool nt;
   int ch;   // register tile width: 4 or 16
@@ -666,6 +681,9 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
   const int64_t elems = (n_in + n_in / 16 + 1) + (square ? 0 : n_out + n_out / 16 + 1);
   lp.lds_bytes = size_t(elems * elem_bytes);
   lp.nt = false;  // measured: the staged runs are faster with normal caching
+  lp.batch = p.batch;
+  tg.bs_in = p.n_in;
+  tg.bs_out = p.n_out;
   lp.ch = std::max(p.din, p.dout) <= 4 ? 4 : 16;
   lp.orb = int((orbits + kBlock - 1) / kBlock);
   if (lp.orb == 3) lp.orb = 4;
@@ -677,8 +695,8 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
 template <typename T, int VEC, int CH, int ORB, class Op>
 static int launch_lds(const LdsPlan& lp, const Op& op, const void* in, void* out,
                       hipStream_t s) {
-  if (lp.tg.n_tiles == 0) return QSV_OK;
-  hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, Op>), dim3(lp.tg.n_tiles),
+  if (lp.tg.n_tiles == 0 || lp.batch == 0) return QSV_OK;
+  hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, Op>), dim3(lp.tg.n_tiles, lp.batch),
                      dim3(kBlock), lp.lds_bytes, s, static_cast<const T*>(in),
                      static_cast<T*>(out), lp.tg, op);
   QSV_CHECK_HIP(hipGetLastError());
@@ -709,8 +727,8 @@ static int dispatch_lds(const LdsPlan& lp, const Op& op, const void* in, void* o
 template <typename T, int CH, int VEC, class Op>
 static int launch_small(const Plan& p, const Op& op, const void* in, void* out,
                         hipStream_t s) {
-  if (p.g.n_threads == 0) return QSV_OK;
-  const dim3 grid(ceil_div_u32(p.g.n_threads, kBlock));
+  if (p.g.n_threads == 0 || p.batch == 0) return QSV_OK;
+  const dim3 grid(ceil_div_u32(p.g.n_threads, kBlock), p.batch);
   if (p.nt)
     hipLaunchKernelGGL((k_apply_small<T, CH, VEC, true, Op>), grid, dim3(kBlock), 0, s,
                        static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
@@ -753,10 +771,11 @@ static int run_inline(const Plan& p, const double* ptm, const void* in, void* ou
 
 template <typename T, int CH>
 static int run_ptr(const Plan& p, const void* ptm_dev, const void* in, void* out,
-                   hipStream_t s) {
+                   hipStream_t s, int64_t pstride = 0) {
   PtrOp<T, CH> op;
   memset(&op, 0, sizeof(op));
   op.ptm = static_cast<const T*>(ptm_dev);
+  op.pstride = pstride;
   op.ld = p.din;
   LdsPlan lp;
   if (make_lds_plan(p, sizeof(T), lp)) return dispatch_lds<T, CH>(lp, op, in, out, s);
@@ -1130,4 +1149,29 @@ extern "C" int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, i
   }
   if (path_out) *path_out = fits_inline(p.din, p.dout) ? 1 : 3;
   return emulate_direct(p, ptm_f64, in, out);
+}
+
+extern "C" int qsv_apply_ptm_batched(int dtype, const void* in, void* out, int batch, int rank,
+                                     const int64_t* dims, int k, const int32_t* targets,
+                                     const int32_t* dims_out_t, const void* ptm_dev,
+                                     int64_t ptm_batch_stride, void* stream) {
+  Plan p;
+  const int rc = make_plan(dtype, rank, dims, k, targets, dims_out_t, p);
+  if (rc) return rc;
+  QSV_CHECK_ARG(batch >= 0 && batch <= 65535, "batch %d outside [0, 65535]", batch);
+  QSV_CHECK_ARG(in && out && ptm_dev, "null pointer argument");
+  QSV_CHECK_ARG(ptm_batch_stride >= 0, "negative PTM batch stride");
+  if (!fits_inline(p.din, p.dout)) {
+    set_error("batched apply supports PTMs up to %d x %d, got %d x %d", kInline, kInline,
+              p.dout, p.din);
+    return QSV_ENOTSUP;
+  }
+  p.batch = uint32_t(batch);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool small = std::max(p.din, p.dout) <= 4;
+  if (dtype == QSV_F64)
+    return small ? run_ptr<double, 4>(p, ptm_dev, in, out, s, ptm_batch_stride)
+                 : run_ptr<double, 16>(p, ptm_dev, in, out, s, ptm_batch_stride);
+  return small ? run_ptr<float, 4>(p, ptm_dev, in, out, s, ptm_batch_stride)
+               : run_ptr<float, 16>(p, ptm_dev, in, out, s, ptm_batch_stride);
 }

@@ -15,6 +15,7 @@ struct Geom {
   FastDiv gdiv[kMaxGroups];
   int64_t gs_in[kMaxGroups];
   int64_t gs_out[kMaxGroups];
+  int64_t bs_in, bs_out;  // element stride between batch entries (grid.y)
 };
 
 template <typename T, int VEC>

@@ -126,3 +126,119 @@ def mixed_qudit_program(dims, n_steps, seed):
                      for q in keep],
         })
     return steps
+
+
+# ---------------------------------------------------------------------------
+# Differentiable, batched noisy circuits (torch, on the device)
+#
+# A layer is: RX(a) then RZ(b) on every qubit, CZ on the brickwork bonds of
+# that layer, then amplitude + phase damping of every qubit for GATE_TIME.
+# Gates are fused per layer before they reach the device: each bond becomes
+# one 2-qubit PTM (D (x) D) CZ (R (x) R), every qubit left out of a bond one
+# 1-qubit PTM D R, so a layer costs n/2 (+1) passes over the state instead
+# of 2.5 n.
+
+def brickwork_pairs(n, layer):
+    return [(q, q + 1) for q in range(layer % 2, n - 1, 2)]
+
+
+def fused_layer_gates(n, layer, rot, damp):
+    """Gates of one layer for a batch: rot, damp [B, n, 4, 4] PTMs."""
+    import torch
+    from .autograd import Gate, cz_ptm, kron_ptm
+    pairs = brickwork_pairs(n, layer)
+    paired = set(q for pq in pairs for q in pq)
+    cz = cz_ptm(rot.device).to(rot.dtype)
+    gates = []
+    for q, q2 in pairs:
+        pre = kron_ptm(rot[:, q], rot[:, q2])
+        post = kron_ptm(damp[:, q], damp[:, q2])
+        gates.append(Gate(post @ cz @ pre, (q, q2)))
+    for q in range(n):
+        if q not in paired:
+            gates.append(Gate(damp[:, q] @ rot[:, q], (q,)))
+    return gates
+
+
+def rotation_ptms(angles):
+    """angles [..., 2] (RX angle, RZ angle) -> PTMs [..., 4, 4] of RZ RX."""
+    from .autograd import rx_ptm, rz_ptm
+    shp = angles.shape[:-1]
+    a = angles.reshape(-1, 2)
+    p = rz_ptm(a[:, 1]) @ rx_ptm(a[:, 0])
+    return p.reshape(*shp, 4, 4)
+
+
+def damping_ptms(t1, t2, t=GATE_TIME * 1e6):
+    """t1, t2 [..., n] in microseconds -> PTMs [..., n, 4, 4]."""
+    from .autograd import damping_ptm
+    shp = t1.shape
+    p = damping_ptm(t1.reshape(-1), t2.reshape(-1), t)
+    return p.reshape(*shp, 4, 4)
+
+
+class TrainableCircuit:
+    """12 qubits, 40 layers, 96 trainable parameters per parameter set:
+    RX/RZ angles shared by the layers of a 5-layer block, three blocks used
+    cyclically (block (layer // 5) mod 3: 3 x 12 x 2 = 72 angles), plus T1
+    and T2 of every qubit (24).  (Eight independent blocks plus T1/T2 would
+    be 216 parameters, not the 96 of the benchmark; the cyclic three-block
+    split is this build's choice.)  The loss is KL(target || simulated) over
+    the 2^n populations; the target comes from the same circuit with
+    seeded, perturbed parameters."""
+
+    def __init__(self, n=12, layers=40, block=5, n_blocks=3):
+        self.n, self.layers, self.block, self.n_blocks = n, layers, block, n_blocks
+        self.n_params = n_blocks * n * 2 + 2 * n
+
+    def initial_params(self, sets, seed, device, dtype):
+        import torch
+        g = np.random.default_rng(seed)
+        ang = g.uniform(0, 2 * np.pi, size=(sets, self.n_blocks, self.n, 2))
+        t1 = g.uniform(20, 40, size=(sets, self.n))
+        t2 = np.array([[g.uniform(10, t1[s, q]) for q in range(self.n)] for s in range(sets)])
+        mk = lambda a: torch.tensor(a, dtype=dtype, device=device)
+        return mk(ang), mk(t1), mk(t2)
+
+    def gates(self, ang, t1, t2):
+        rot = rotation_ptms(ang)                   # [S, blocks, n, 4, 4]
+        damp = damping_ptms(t1, t2)                # [S, n, 4, 4]
+        out = []
+        for layer in range(self.layers):
+            b = (layer // self.block) % self.n_blocks
+            out += fused_layer_gates(self.n, layer, rot[:, b], damp)
+        return out
+
+    def populations(self, ang, t1, t2, dtype, checkpoint_every=8):
+        import torch
+        from .autograd import run_circuit, diagonal
+        from .bases import general
+        S = ang.shape[0]
+        N = 4 ** self.n
+        x0 = torch.zeros((S, N), dtype=dtype, device=ang.device)
+        x0[:, 0] = 1
+        gates = self.gates(ang, t1, t2)
+        x = run_circuit(x0, gates, (4,) * self.n, checkpoint_every)
+        return diagonal(x, [general(2)] * self.n)
+
+    @staticmethod
+    def kl(target, p, eps=1e-30):
+        import torch
+        t = target.clamp_min(eps)
+        return (target * (torch.log(t) - torch.log(p.clamp_min(eps)))).sum(-1)
+
+
+def random_circuit_batch(n_circuits, seed, n_range=(4, 11), layer_range=(20, 80)):
+    """The batched sweep: per circuit a qubit count ~ U{4..11}, a layer count
+    ~ U{20..80}, RX/RZ angles ~ U(0, 2 pi) for every layer and qubit, and
+    T1 ~ U(20, 40) us, T2 ~ U(10, T1) us per qubit."""
+    g = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_circuits):
+        n = int(g.integers(n_range[0], n_range[1] + 1))
+        L = int(g.integers(layer_range[0], layer_range[1] + 1))
+        t1 = g.uniform(20, 40, size=n)
+        t2 = np.array([g.uniform(10, t1[q]) for q in range(n)])
+        out.append({'n': n, 'layers': L, 'angles': g.uniform(0, 2 * np.pi, size=(L, n, 2)),
+                    't1': t1, 't2': t2})
+    return out
