@@ -44,6 +44,10 @@ def parse_args():
                     help='seconds of numpy work for the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--qubits', type=int, default=14)
+    ap.add_argument('--no-extras', action='store_true',
+                    help='skip the batched and differentiable measurements')
+    ap.add_argument('--circuits', type=int, default=512)
+    ap.add_argument('--param-sets', type=int, default=64)
     return ap.parse_args()
 
 
@@ -100,6 +104,86 @@ def cpu_baseline(n, pairs, ptm, budget):
                                                os.environ.get('OPENBLAS_NUM_THREADS', 'unset')),
         'seconds_per_app': [round(x, 3) for x in times],
     }
+
+
+def bench_batched(args):
+    """512 circuits (4..11 qubits, 20..80 layers, float32 states, float64
+    readout): one HIP-graph replay of the batched forward + adjoint against
+    one-circuit-at-a-time eager runs of a stratified subsample."""
+    import numpy as np
+    import torch
+    from quantumsim_amd.batch import CircuitBatch, plain_forward_backward
+    from quantumsim_amd.workloads import random_circuit_batch
+    circuits = random_circuit_batch(args.circuits, seed=2024)
+    t0 = time.perf_counter()
+    cb = CircuitBatch(circuits, dtype=torch.float32, checkpoint_every=16)
+    w = cb.random_weights(7)
+    cb.capture(w)
+    t_setup = time.perf_counter() - t0
+    stream = cb.stream
+    reps = 3
+    sec = timed(lambda: cb.graph.replay(), reps, stream)
+    log('batched: %d circuits fwd+adjoint %.3f s per step (setup+capture %.1f s)'
+        % (len(circuits), sec, t_setup))
+    # plain baseline: each circuit alone, eager launches, same kernels
+    by_n = {}
+    for i, c in enumerate(circuits):
+        by_n.setdefault(c['n'], []).append(i)
+    sample = [by_n[n][j] for n in sorted(by_n) for j in range(min(2, len(by_n[n])))]
+    weights = np.random.default_rng(7)
+    per = []
+    for i in sample:
+        c = circuits[i]
+        wt = torch.tensor(weights.standard_normal((1, 2 ** c['n'])), dtype=torch.float64,
+                          device='cuda')
+        plain_forward_backward(c, wt)                  # warm-up of this shape
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plain_forward_backward(c, wt)
+        torch.cuda.synchronize()
+        per.append(time.perf_counter() - t1)
+    # weight the per-width averages by the widths' share of the batch
+    mean_by_n = {}
+    for i, t in zip(sample, per):
+        mean_by_n.setdefault(circuits[i]['n'], []).append(t)
+    plain_sec = sum(len(by_n[n]) * statistics.mean(v) for n, v in mean_by_n.items())
+    res = {'circuits': len(circuits), 'fwd_adjoint_s': round(sec, 4),
+           'circuits_per_s': round(len(circuits) / sec, 1),
+           'plain_circuits_per_s': round(len(circuits) / plain_sec, 2),
+           'speedup_vs_plain': round(plain_sec / sec, 2),
+           'plain_sample': len(sample), 'dtype': 'f32 state, f64 readout',
+           'checkpoint_every': 16}
+    log('batched: %s' % res)
+    del cb
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_differentiable(args):
+    """12 qubits, 40 layers, 96 parameters x 64 parameter sets, float32
+    states: KL loss forward + adjoint (checkpoint every 8 gates)."""
+    import torch
+    from quantumsim_amd.workloads import TrainableCircuit
+    tc = TrainableCircuit(n=12, layers=40)
+    sets = args.param_sets
+    ang, t1, t2 = tc.initial_params(sets, seed=11, device='cuda', dtype=torch.float64)
+    with torch.no_grad():
+        target = tc.populations(ang + 0.1, t1 * 1.05, t2 * 0.97, torch.float32)
+    params = [ang.requires_grad_(), t1.requires_grad_(), t2.requires_grad_()]
+
+    def step():
+        p = tc.populations(*params, torch.float32, checkpoint_every=8)
+        loss = tc.kl(target, p).sum()
+        return torch.autograd.grad(loss, params)
+
+    step()
+    torch.cuda.synchronize()
+    sec = timed(step, 2, torch.cuda.current_stream())
+    res = {'qubits': 12, 'layers': 40, 'params_per_set': tc.n_params, 'param_sets': sets,
+           'fwd_adjoint_s': round(sec, 4), 'param_sets_per_s': round(sets / sec, 2),
+           'gates': len(tc.gates(*[p.detach()[:1] for p in params])), 'checkpoint_every': 8}
+    log('differentiable: %s' % res)
+    return res
 
 
 def main():
@@ -215,6 +299,9 @@ def main():
         'diagonal_us': {'f64': round(r64['diag_us'], 1),
                         'f32': round(res['float32']['diag_us'], 1)},
     }
+    if not args.no_extras:
+        out['batched'] = bench_batched(args)
+        out['differentiable'] = bench_differentiable(args)
     if not args.no_cpu:
         log('cpu baseline (numpy oracle, budget %.0f s)...' % args.cpu_budget)
         cpu_pairs = [(0, 1), (n // 2 - 1, n // 2), (n - 2, n - 1)]

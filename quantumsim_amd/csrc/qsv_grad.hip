@@ -6,10 +6,9 @@
 // loss with respect to the result.  (The gradient with respect to x is the
 // same contraction with the transposed PTM and reuses the apply kernels.)
 //
-// Each thread walks orbits with a grid-stride loop and accumulates a 4 x CH
-// block of dP (4 output rows per grid.z slice) in float64; waves reduce with
-// shuffles, blocks write float64 partials, a second kernel sums the partials
-// in a fixed order.
+// Blocks stage chunks of orbits in LDS and accumulate the outer products in
+// float64 registers; blocks write float64 partials, a second kernel sums the
+// partials in a fixed order (deterministic).
 #include "qsv_geom.h"
 
 #include <string.h>
@@ -17,7 +16,6 @@
 
 namespace qsv {
 
-constexpr int kGradRows = 4;
 constexpr int kGradBlocks = 256;
 
 struct GradOffs {
@@ -25,62 +23,67 @@ struct GradOffs {
   int64_t off_out[16];
 };
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-template <typename T, int CH>
+// A block walks chunks of CHUNK orbits: thread t gathers orbit t's D_in
+// entries of x and D_out entries of ybar into LDS (orbit-major rows, so the
+// gathers of neighbouring lanes hit neighbouring orbits), then wave w
+// accumulates the outer products of its quarter of the chunk, lane l owning
+// dP[o = l / 4][4 (l % 4) .. 4 (l % 4) + 3] in float64.  x and ybar are read
+// once.  The four waves' sums meet in LDS at the end.
+template <typename T, int CHUNK>
 __global__ void __launch_bounds__(kBlock)
     k_ptm_grad(const T* __restrict__ x, const T* __restrict__ ybar, const Geom g,
                const GradOffs offs, const uint32_t n_orbits, double* __restrict__ partial) {
+  constexpr int LD = 17;  // padded row of 16
+  __shared__ T X[CHUNK * LD];
+  __shared__ T Y[CHUNK * LD];
+  __shared__ double red[kBlock / 64][64 * 4];
   const uint32_t b = blockIdx.y;
-  const int o0 = blockIdx.z * kGradRows;
   x += int64_t(b) * g.bs_in;
   ybar += int64_t(b) * g.bs_out;
-  double acc[kGradRows][CH];
-#pragma unroll
-  for (int r = 0; r < kGradRows; ++r)
-#pragma unroll
-    for (int i = 0; i < CH; ++i) acc[r][i] = 0.0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int o = lane >> 2, i0 = (lane & 3) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  constexpr int PER_WAVE = CHUNK / (kBlock / 64);
 
-  const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < n_orbits; m += stride) {
-    int64_t bin, bout;
-    orbit_base(g, m, bin, bout);
-    T xv[CH];
+  for (uint32_t c0 = blockIdx.x * uint32_t(CHUNK); c0 < n_orbits;
+       c0 += gridDim.x * uint32_t(CHUNK)) {
+    for (int j = tid; j < CHUNK; j += kBlock) {
+      const uint32_t m = c0 + j;
+      if (m < n_orbits) {
+        int64_t bin, bout;
+        orbit_base(g, m, bin, bout);
 #pragma unroll
-    for (int i = 0; i < CH; ++i) xv[i] = i < g.din ? x[bin + offs.off_in[i]] : T(0);
+        for (int i = 0; i < 16; ++i) X[j * LD + i] = i < g.din ? x[bin + offs.off_in[i]] : T(0);
 #pragma unroll
-    for (int r = 0; r < kGradRows; ++r) {
-      if (o0 + r < g.dout) {
-        const double yv = double(ybar[bout + offs.off_out[o0 + r]]);
+        for (int r = 0; r < 16; ++r)
+          Y[j * LD + r] = r < g.dout ? ybar[bout + offs.off_out[r]] : T(0);
+      } else {
 #pragma unroll
-        for (int i = 0; i < CH; ++i) acc[r][i] += yv * double(xv[i]);
+        for (int i = 0; i < 16; ++i) X[j * LD + i] = Y[j * LD + i] = T(0);
       }
     }
+    __syncthreads();
+    const int j0 = w * PER_WAVE;
+#pragma unroll 4
+    for (int j = j0; j < j0 + PER_WAVE; ++j) {
+      const double yv = double(Y[j * LD + o]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] += yv * double(X[j * LD + i0 + c]);
+    }
+    __syncthreads();
   }
-
-  __shared__ double red[kBlock / 64][kGradRows * CH];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int r = 0; r < kGradRows; ++r)
-#pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      const double s = wave_sum_d(acc[r][i]);
-      if (lane == 0) red[wave][r * CH + i] = s;
-    }
+  for (int c = 0; c < 4; ++c) red[w][lane * 4 + c] = acc[c];
   __syncthreads();
-  if (threadIdx.x < kGradRows * CH) {
-    const int r = threadIdx.x / CH, i = threadIdx.x % CH;
-    if (o0 + r < g.dout && i < g.din) {
-      double s = 0.0;
+  // thread t < 256 sums entry t = (o, i) over the four waves
+  {
+    const int oo = tid >> 4, ii = tid & 15;
+    const int src = (oo * 4 + (ii >> 2)) * 4 + (ii & 3);  // lane (oo, ii / 4), slot ii % 4
+    double s = 0.0;
 #pragma unroll
-      for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
-      // partial[b][block][o][i]
-      partial[((int64_t(b) * gridDim.x + blockIdx.x) * g.dout + (o0 + r)) * g.din + i] = s;
-    }
+    for (int ww = 0; ww < kBlock / 64; ++ww) s += red[ww][src];
+    if (oo < g.dout && ii < g.din)
+      partial[((int64_t(b) * gridDim.x + blockIdx.x) * g.dout + oo) * g.din + ii] = s;
   }
 }
 
@@ -200,7 +203,7 @@ static int make_grad_plan(int rank, const int64_t* dims, int k, const int32_t* t
 }
 
 static int grad_blocks(uint32_t n_orbits) {
-  const uint32_t b = (n_orbits + kBlock * 4 - 1) / (kBlock * 4);
+  const uint32_t b = (n_orbits + kBlock * 2 - 1) / (kBlock * 2);
   return int(b < 1 ? 1 : (b > kGradBlocks ? kGradBlocks : b));
 }
 
@@ -234,27 +237,15 @@ extern "C" int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batc
                 (long long)need);
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* partial = static_cast<double*>(workspace);
-  const dim3 grid(nb, batch, (p.dout + kGradRows - 1) / kGradRows);
-  const bool small = std::max(p.din, p.dout) <= 4;
-  if (dtype == QSV_F64) {
-    if (small)
-      hipLaunchKernelGGL((k_ptm_grad<double, 4>), grid, dim3(kBlock), 0, s,
-                         static_cast<const double*>(x), static_cast<const double*>(ybar), p.g,
-                         p.offs, p.n_orbits, partial);
-    else
-      hipLaunchKernelGGL((k_ptm_grad<double, 16>), grid, dim3(kBlock), 0, s,
-                         static_cast<const double*>(x), static_cast<const double*>(ybar), p.g,
-                         p.offs, p.n_orbits, partial);
-  } else {
-    if (small)
-      hipLaunchKernelGGL((k_ptm_grad<float, 4>), grid, dim3(kBlock), 0, s,
-                         static_cast<const float*>(x), static_cast<const float*>(ybar), p.g,
-                         p.offs, p.n_orbits, partial);
-    else
-      hipLaunchKernelGGL((k_ptm_grad<float, 16>), grid, dim3(kBlock), 0, s,
-                         static_cast<const float*>(x), static_cast<const float*>(ybar), p.g,
-                         p.offs, p.n_orbits, partial);
-  }
+  const dim3 grid(nb, batch);
+  if (dtype == QSV_F64)
+    hipLaunchKernelGGL((k_ptm_grad<double, 128>), grid, dim3(kBlock), 0, s,
+                       static_cast<const double*>(x), static_cast<const double*>(ybar), p.g,
+                       p.offs, p.n_orbits, partial);
+  else
+    hipLaunchKernelGGL((k_ptm_grad<float, 256>), grid, dim3(kBlock), 0, s,
+                       static_cast<const float*>(x), static_cast<const float*>(ybar), p.g,
+                       p.offs, p.n_orbits, partial);
   QSV_CHECK_HIP(hipGetLastError());
   const int dd = p.din * p.dout;
   const dim3 g2((dd + kBlock - 1) / kBlock, batch);

@@ -1,0 +1,82 @@
+"""Workloads for rocprofv3 runs (kernel trace, FETCH_SIZE / WRITE_SIZE).
+
+    python -m quantumsim_amd.profiling apply2q --qubits 14 --dtype float64 --reps 3
+    python -m quantumsim_amd.profiling fused --qubits 14 --dtype float64 --reps 3
+
+Every run first copies a buffer of known size (2 * 4^n * element bytes of
+traffic, torch's copy kernel) so that HBM counters can be scaled against a
+plain copy, then runs the kernels under study.
+  apply2q : the 13 two-qubit PTM applications of one brickwork layer pair
+  apply1q : one-qubit PTM applications on qubits 0, n/2, n-1
+  fused   : dense 3- and 4-qubit PTMs (MFMA kernel) on middle and inner qubits
+  batched : one forward + adjoint step of the 512-circuit batch
+"""
+import argparse
+import os
+import sys
+
+import numpy as np
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument('what', choices=['apply2q', 'apply1q', 'fused', 'batched'])
+    ap.add_argument('--qubits', type=int, default=14)
+    ap.add_argument('--dtype', default='float64')
+    ap.add_argument('--reps', type=int, default=3)
+    a = ap.parse_args(argv)
+    import torch
+    import quantumsim_amd as qa
+    from quantumsim_amd import bases
+    from quantumsim_amd.workloads import noisy_layers
+
+    n = a.qubits
+    N = 4 ** n
+    torch.cuda.set_device(0)
+    dt = torch.float64 if a.dtype == 'float64' else torch.float32
+    src = torch.ones(N, dtype=dt, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    print('copy reference: %d bytes read, %d bytes written' % (N * src_bytes(dt), N * src_bytes(dt)),
+          flush=True)
+    if a.what == 'batched':
+        from quantumsim_amd.batch import CircuitBatch
+        from quantumsim_amd.workloads import random_circuit_batch
+        cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32)
+        w = cb.random_weights(7)
+        for _ in range(a.reps):
+            cb.step(w)
+        torch.cuda.synchronize()
+        print('done', flush=True)
+        return 0
+    v = qa.PauliVectorHIP([bases.general(2)] * n, force=True, dtype=dt)
+    v.to_torch().view(-1).copy_(torch.randn(N, dtype=dt, device='cuda'))
+    circuit, _ = noisy_layers(n, 2, seed=1, pairing='brickwork')
+    if a.what == 'apply2q':
+        gates = [(p, q) for p, q in circuit if len(q) == 2]
+    elif a.what == 'apply1q':
+        p1 = circuit[0][0]
+        gates = [(p1, (q,)) for q in (0, n // 2, n - 1)]
+    else:
+        rng = np.random.default_rng(0)
+        gates = [(rng.standard_normal([4] * 6) / 64, (5, 6, 7)),
+                 (rng.standard_normal([4] * 6) / 64, (n - 3, n - 2, n - 1)),
+                 (rng.standard_normal([4] * 8) / 256, (5, 6, 7, 8)),
+                 (rng.standard_normal([4] * 8) / 256, (n - 4, n - 3, n - 2, n - 1))]
+    for _ in range(a.reps):
+        for p, q in gates:
+            v.apply_ptm(p, *q)
+    torch.cuda.synchronize()
+    print('done: %d launches per rep' % len(gates), flush=True)
+    return 0
+
+
+def src_bytes(dt):
+    import torch
+    return 8 if dt == torch.float64 else 4
+
+
+if __name__ == '__main__':
+    sys.exit(main())
