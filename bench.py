@@ -281,7 +281,7 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': round(r64['gbs'], 1), 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': round(r64['gbs'] / HBM_PEAK_GBS, 4),
                      'traffic': load_traffic(),
-                     'kernel': 'k_apply_small<double,16,*> (2-qubit apply)',
+                     'kernel': '2-qubit apply: k_apply_small (outer/middle bonds), k_apply_lds (inner bonds)',
                      'alg_bytes_per_launch': r64['alg_bytes'],
                      'avg_launch_ms': round(r64['per_launch'] * 1e3, 4)},
         'f32': {'apps_per_s': round(res['float32']['apps'], 2),
