@@ -211,7 +211,7 @@ def main():
     res = {}
     for dtype, es in (('float64', 8), ('float32', 4)):
         v = qa.PauliVectorHIP([bases.general(2)] * n, force=True, dtype=dtype)
-        v._data.copy_(torch.randn(N, dtype=v.dtype, device=v.device))
+        v.to_torch().copy_(torch.randn((4,) * n, dtype=v.dtype, device=v.device))
 
         def step():
             for p, q in step_gates:

@@ -64,6 +64,18 @@ int qsv_apply_ptm(int dtype, const void* in, void* out, int rank,
                   const int32_t* dims_out_t, const double* ptm_f64,
                   void* workspace, int64_t workspace_bytes, void* stream);
 
+/* qsv_apply_ptm on strided tensors: in_strides[rank] / out_strides[rank]
+ * are element strides per axis (NULL = C-contiguous) in C order without
+ * overlap (stride[q] >= stride[q+1] * dims[q+1]); gaps are allowed.  The
+ * backend pads the strides of the outermost axes of large vectors so that
+ * the operand streams of a gate on two outer qudits do not alias in the
+ * HBM channel/bank mapping. */
+int qsv_apply_ptm_strided(int dtype, const void* in, void* out, int rank,
+                          const int64_t* dims, const int64_t* in_strides,
+                          const int64_t* out_strides, int k, const int32_t* targets,
+                          const int32_t* dims_out_t, const double* ptm_f64,
+                          void* workspace, int64_t workspace_bytes, void* stream);
+
 /* Same contraction with the PTM already on the device, C-order
  * (out..., in...), in the state's element type (used when PTMs are built
  * on the device, e.g. from trainable parameters, and under graph capture).
@@ -117,6 +129,13 @@ int qsv_multitake(int in_dtype, int out_dtype, const void* in, void* out,
                   const int32_t* idx_dev, const int32_t* offsets,
                   void* stream);
 
+/* qsv_multitake on a strided input (in_strides[rank] element strides, NULL
+ * = C-contiguous; C order without overlap, gaps allowed). */
+int qsv_multitake_strided(int in_dtype, int out_dtype, const void* in, void* out, int rank,
+                          const int64_t* in_dims, const int64_t* in_strides,
+                          const int64_t* out_dims, const int32_t* idx_dev,
+                          const int32_t* offsets, void* stream);
+
 /* Bytes of device workspace qsv_axis_sum needs. */
 int64_t qsv_axis_sum_workspace_bytes(int64_t n_outer, int d, int64_t n_inner);
 
@@ -150,11 +169,13 @@ int64_t qsv_selftest_fastdiv(int32_t n_divisors, uint64_t seed);
  * (reported in *path_out: 1 direct, 2 LDS-tiled, 3 table, 4 MFMA dense), then every
  * thread's address arithmetic is executed on the host in float64 with each
  * global and LDS access checked against its buffer and every output entry
- * checked to be written exactly once.  in/out/ptm are host float64 arrays.
+ * checked to be written exactly once (padding never).  in/out/ptm are host
+ * float64 arrays of the strided extents (NULL strides = contiguous).
  * Returns QSV_EINVAL with a message on any violation.  No counterpart in
  * the reference; it lets CPU tests pin the kernels' indexing before GPU use. */
 int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, int rank,
-                          const int64_t* dims, int k, const int32_t* targets,
+                          const int64_t* dims, const int64_t* in_strides,
+                          const int64_t* out_strides, int k, const int32_t* targets,
                           const int32_t* dims_out_t, const double* ptm_f64,
                           int32_t* path_out);
 

@@ -38,6 +38,9 @@ _SIGNATURES = {
     'qsv_apply_ptm': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, _c_i64p,
                                      ctypes.c_int, _c_i32p, _c_i32p, _c_f64p, _vp,
                                      ctypes.c_int64, _vp]),
+    'qsv_apply_ptm_strided': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, _c_i64p,
+                                             _c_i64p, _c_i64p, ctypes.c_int, _c_i32p, _c_i32p,
+                                             _c_f64p, _vp, ctypes.c_int64, _vp]),
     'qsv_apply_ptm_dev': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, _c_i64p,
                                          ctypes.c_int, _c_i32p, _c_i32p, _vp, _vp,
                                          ctypes.c_int64, _vp]),
@@ -51,6 +54,8 @@ _SIGNATURES = {
                                     ctypes.c_int64, _vp]),
     'qsv_multitake': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                      _c_i64p, _c_i64p, _vp, _c_i32p, _vp]),
+    'qsv_multitake_strided': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                             _c_i64p, _c_i64p, _c_i64p, _vp, _c_i32p, _vp]),
     'qsv_axis_sum_workspace_bytes': (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int,
                                                       ctypes.c_int64]),
     'qsv_axis_sum': (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_int,
@@ -58,8 +63,8 @@ _SIGNATURES = {
     'qsv_scale': (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int64, ctypes.c_double, _vp]),
     'qsv_selftest_fastdiv': (ctypes.c_int64, [ctypes.c_int32, ctypes.c_uint64]),
     'qsv_emulate_apply_ptm': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int,
-                                             _c_i64p, ctypes.c_int, _c_i32p, _c_i32p, _c_f64p,
-                                             _c_i32p]),
+                                             _c_i64p, _c_i64p, _c_i64p, ctypes.c_int, _c_i32p,
+                                             _c_i32p, _c_f64p, _c_i32p]),
 }
 
 _lib = None

@@ -71,7 +71,7 @@ struct TableOp {
 
 // ---------------------------------------------------------------- small
 // din, dout <= CH: all D_in operands live in registers, outputs stream out.
-template <typename T, int CH, int VEC, bool NT, class Op>
+template <typename T, int CH, int VEC, bool NT, bool FULL, class Op>
 __global__ void __launch_bounds__(kBlock)
     k_apply_small(const T* __restrict__ in, T* __restrict__ out, const Geom g, const Op op) {
   typedef typename VT<T, VEC>::t V;
@@ -83,21 +83,23 @@ __global__ void __launch_bounds__(kBlock)
   int64_t bin, bout;
   orbit_base(g, t * VEC, bin, bout);
 
+  // FULL: din == dout == CH, no per-index guards (all CH loads issue back
+  // to back in one basic block)
   V v[CH];
 #pragma unroll
   for (int i = 0; i < CH; ++i) {
-    if (i < g.din)
+    if (FULL || i < g.din)
       v[i] = gload<NT>(reinterpret_cast<const V*>(in + bin + op.oin(i)));
     else
       v[i] = V(0);
   }
 #pragma unroll
   for (int o = 0; o < CH; ++o) {
-    if (o < g.dout) {
+    if (FULL || o < g.dout) {
       V acc = V(0);
 #pragma unroll
       for (int i = 0; i < CH; ++i) {
-        if (Op::kPadded || i < g.din) acc += op.p(o, i, b) * v[i];
+        if (FULL || Op::kPadded || i < g.din) acc += op.p(o, i, b) * v[i];
       }
       gstore<NT>(reinterpret_cast<V*>(out + bout + op.oout(o)), acc);
     }
@@ -225,7 +227,7 @@ __host__ __device__ __forceinline__ void tile_orbit_base(const TileGeom& tg, int
   }
 }
 
-template <typename T, int VEC, bool NT, int CH, int ORB, class Op>
+template <typename T, int VEC, bool NT, int CH, int ORB, bool FULL, class Op>
 __global__ void __launch_bounds__(kBlock)
     k_apply_lds(const T* __restrict__ in, T* __restrict__ out, const TileGeom tg, const Op op) {
   typedef typename VT<T, VEC>::t V;
@@ -268,7 +270,7 @@ __global__ void __launch_bounds__(kBlock)
       tile_orbit_base(tg, j, bin, bout[r]);
 #pragma unroll
       for (int i = 0; i < CH; ++i)
-        v[r][i] = i < tg.din ? lin[lds_pad(bin + tg.slot_in[i])] : T(0);
+        v[r][i] = (FULL || i < tg.din) ? lin[lds_pad(bin + tg.slot_in[i])] : T(0);
     }
   }
   if (tg.in_place) __syncthreads();
@@ -277,11 +279,11 @@ __global__ void __launch_bounds__(kBlock)
     if (tid + r * kBlock < tg.orbits) {
 #pragma unroll
       for (int o = 0; o < CH; ++o) {
-        if (o < tg.dout) {
+        if (FULL || o < tg.dout) {
           T acc = T(0);
 #pragma unroll
           for (int i = 0; i < CH; ++i) {
-            if (Op::kPadded || i < tg.din) acc += op.p(o, i, b) * v[r][i];
+            if (FULL || Op::kPadded || i < tg.din) acc += op.p(o, i, b) * v[r][i];
           }
           lout[lds_pad(bout[r] + tg.slot_out[o])] = acc;
         }
@@ -334,7 +336,11 @@ struct Plan {
   bool is_target[QSV_MAX_RANK];
 };
 
-static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int32_t* targets,
+// in_strides / out_strides: element strides per axis (NULL = C-contiguous).
+// Strided layouts must keep C order without overlap: stride[q] >=
+// stride[q+1] * dims[q+1]; gaps (padding) are allowed.
+static int make_plan(int dtype, int rank, const int64_t* dims, const int64_t* in_strides,
+                     const int64_t* out_strides, int k, const int32_t* targets,
                      const int32_t* dims_out_t, Plan& p) {
   QSV_CHECK_ARG(dtype == QSV_F32 || dtype == QSV_F64, "unknown dtype %d", dtype);
   QSV_CHECK_ARG(rank >= 0 && rank <= QSV_MAX_RANK, "rank %d outside [0, %d]", rank,
@@ -363,13 +369,16 @@ static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int3
   int64_t sin[QSV_MAX_RANK], sout[QSV_MAX_RANK];
   int64_t acc_in = 1, acc_out = 1;
   for (int q = rank - 1; q >= 0; --q) {
-    sin[q] = acc_in;
-    sout[q] = acc_out;
-    acc_in *= dims[q];
-    acc_out *= dout_full[q];
+    sin[q] = in_strides ? in_strides[q] : acc_in;
+    sout[q] = out_strides ? out_strides[q] : acc_out;
+    QSV_CHECK_ARG(sin[q] >= acc_in && sout[q] >= acc_out,
+                  "strides of axis %d overlap the axes below it", q);
+    acc_in = sin[q] * dims[q];
+    acc_out = sout[q] * dout_full[q];
     QSV_CHECK_ARG(acc_in < (int64_t(1) << 40) && acc_out < (int64_t(1) << 40),
                   "Pauli vector too large");
   }
+  // extents: one past the largest address
   p.n_in = acc_in;
   p.n_out = acc_out;
   // vectors that cannot stay resident in the last-level caches between
@@ -426,18 +435,20 @@ static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int3
   int64_t M = 1;
   int q = rank - 1;
   while (q >= 0) {
-    if (is_target[q]) {
+    if (is_target[q] || dims[q] == 1) {
       --q;
       continue;
     }
-    int64_t size = 1;
+    // a group: adjacent bystander axes whose strides nest without gaps
+    int64_t size = dims[q];
     const int64_t s_in = sin[q], s_out = sout[q];
-    while (q >= 0 && !is_target[q]) {
+    --q;
+    while (q >= 0 && !is_target[q] && sin[q] == s_in * size && sout[q] == s_out * size) {
       size *= dims[q];
       --q;
     }
-    if (size == 1) continue;
     QSV_CHECK_ARG(size < (int64_t(1) << 32), "bystander group too large");
+    QSV_CHECK_ARG(ng < kMaxGroups, "too many bystander groups");
     g.gs_in[ng] = s_in;
     g.gs_out[ng] = s_out;
     g.gdiv[ng] = make_fastdiv(uint32_t(size));
@@ -453,11 +464,16 @@ static int make_plan(int dtype, int rank, const int64_t* dims, int k, const int3
   // vector accesses stay aligned.
   int vec = 1;
   if (ng > 0 && g.gs_in[0] == 1 && g.gs_out[0] == 1) {
-    const uint32_t g0 = g.gdiv[0].d;
     // f64: 2 x 8 B; f32: 4 x 4 B (a float2 variant needs 224 AGPRs on gfx950
-    // and is not instantiated)
+    // and is not instantiated).  Every other stride and target offset must
+    // be a multiple of the width for the vector accesses to stay aligned.
     const int want = dtype == QSV_F64 ? 2 : 4;
-    vec = (g0 % want) == 0 ? want : 1;
+    bool aligned = g.gdiv[0].d % want == 0;
+    for (int j = 1; j < ng; ++j)
+      aligned = aligned && g.gs_in[j] % want == 0 && g.gs_out[j] % want == 0;
+    for (int i = 0; i < p.din; ++i) aligned = aligned && p.off_in[i] % want == 0;
+    for (int o = 0; o < p.dout; ++o) aligned = aligned && p.off_out[o] % want == 0;
+    vec = aligned ? want : 1;
   }
   p.vec = vec;
   QSV_CHECK_ARG(M / vec < (int64_t(1) << 32) - kBlock, "too many orbits for 32-bit indexing");
@@ -513,6 +529,14 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
   int64_t Z = 1;
   for (int q = t_in + 1; q < p.rank; ++q) Z *= p.dims_in[q];
   if (Z * elem_bytes >= lds_run_threshold()) return false;
+  // axis c nests without a gap on the axes below it (input and output)
+  auto contig = [&](int c) {
+    if (c == p.rank - 1) return p.sin[c] == 1 && p.sout[c] == 1;
+    return p.sin[c] == p.sin[c + 1] * p.dims_in[c + 1] &&
+           p.sout[c] == p.sout[c + 1] * p.dims_out[c + 1];
+  };
+  for (int q = t_in; q < p.rank; ++q)
+    if (!contig(q)) return false;
 
   TileGeom& tg = lp.tg;
   memset(&tg, 0, sizeof(tg));
@@ -521,6 +545,7 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
   int64_t P = Z;  // orbits per region block
   while (c0 > 0) {
     const int c = c0 - 1;
+    if (!contig(c)) break;
     const int64_t np = p.is_target[c] ? P : P * p.dims_in[c];
     if (np > cap) break;
     P = np;
@@ -533,8 +558,8 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
   }
   // chunk of the bystander axis right above the region
   int64_t Tx = 1;
-  if (c0 > 0) {
-    const int64_t d = p.dims_in[c0 - 1];  // a bystander (targets were absorbed)
+  if (c0 > 0 && !p.is_target[c0 - 1] && contig(c0 - 1)) {
+    const int64_t d = p.dims_in[c0 - 1];  // a bystander next to the region
     for (int64_t t = d; t >= 1; --t)
       if (d % t == 0 && P * t <= cap) {
         Tx = t;
@@ -639,26 +664,29 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
     int ng = 0;
     int64_t n_tiles = 1;
     int q = c0 - 1;
-    bool first = true;
     while (q >= 0) {
-      if (p.is_target[q]) {
+      if (p.is_target[q] || (p.dims_in[q] == 1 && q != c0 - 1)) {
         --q;
         continue;
       }
-      int64_t size = 1;
+      // first axis of the group (the chunked axis contributes dims / Tx at
+      // stride Tx * s); more axes join while their strides nest
+      int64_t d0 = p.dims_in[q];
       int64_t s_in = p.sin[q], s_out = p.sout[q];
-      while (q >= 0 && !p.is_target[q]) {
-        int64_t d = p.dims_in[q];
-        if (first && q == c0 - 1) {
-          d /= Tx;
-          s_in *= Tx;
-          s_out *= Tx;
-        }
-        size *= d;
+      if (q == c0 - 1) {
+        d0 /= Tx;
+        s_in *= Tx;
+        s_out *= Tx;
+      }
+      int64_t size = d0;
+      --q;
+      while (q >= 0 && !p.is_target[q] && p.sin[q] == s_in * size &&
+             p.sout[q] == s_out * size) {
+        size *= p.dims_in[q];
         --q;
       }
-      first = false;
       if (size == 1) continue;
+      if (ng >= kMaxGroups) return false;
       if (size >= (int64_t(1) << 32)) return false;
       tg.tdiv[ng] = make_fastdiv(uint32_t(size));
       tg.ts_in[ng] = s_in;
@@ -675,8 +703,15 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
   for (int j = 0; j < p.k; ++j)
     square = square && p.dims_in[p.targets[j]] == p.dims_out[p.targets[j]];
   tg.in_place = square ? 1 : 0;
+  // vector width: the run lengths, run offsets and tile strides must all be
+  // multiples of it (padded layouts may break this)
   const int want = elem_bytes == 8 ? 2 : 4;
-  lp.vec = (R_in % want == 0 && R_out % want == 0) ? want : 1;
+  bool aligned = R_in % want == 0 && R_out % want == 0;
+  for (int a = 0; a < tg.runs_in; ++a) aligned = aligned && tg.run_in[a] % want == 0;
+  for (int a = 0; a < tg.runs_out; ++a) aligned = aligned && tg.run_out[a] % want == 0;
+  for (int j = 0; j < tg.ntg; ++j)
+    aligned = aligned && tg.ts_in[j] % want == 0 && tg.ts_out[j] % want == 0;
+  lp.vec = aligned ? want : 1;
   const int64_t n_in = int64_t(tg.runs_in) * tg.L_in, n_out = int64_t(tg.runs_out) * tg.L_out;
   const int64_t elems = (n_in + n_in / 16 + 1) + (square ? 0 : n_out + n_out / 16 + 1);
   lp.lds_bytes = size_t(elems * elem_bytes);
@@ -696,9 +731,14 @@ template <typename T, int VEC, int CH, int ORB, class Op>
 static int launch_lds(const LdsPlan& lp, const Op& op, const void* in, void* out,
                       hipStream_t s) {
   if (lp.tg.n_tiles == 0 || lp.batch == 0) return QSV_OK;
-  hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, Op>), dim3(lp.tg.n_tiles, lp.batch),
-                     dim3(kBlock), lp.lds_bytes, s, static_cast<const T*>(in),
-                     static_cast<T*>(out), lp.tg, op);
+  if (lp.tg.din == CH && lp.tg.dout == CH)
+    hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, true, Op>),
+                       dim3(lp.tg.n_tiles, lp.batch), dim3(kBlock), lp.lds_bytes, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), lp.tg, op);
+  else
+    hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, false, Op>),
+                       dim3(lp.tg.n_tiles, lp.batch), dim3(kBlock), lp.lds_bytes, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), lp.tg, op);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
 }
@@ -729,12 +769,19 @@ static int launch_small(const Plan& p, const Op& op, const void* in, void* out,
                         hipStream_t s) {
   if (p.g.n_threads == 0 || p.batch == 0) return QSV_OK;
   const dim3 grid(ceil_div_u32(p.g.n_threads, kBlock), p.batch);
-  if (p.nt)
-    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, true, Op>), grid, dim3(kBlock), 0, s,
+  const bool full = p.din == CH && p.dout == CH;
+  if (p.nt && full)
+    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, true, true, Op>), grid, dim3(kBlock), 0, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
+  else if (p.nt)
+    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, true, false, Op>), grid, dim3(kBlock), 0, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
+  else if (full)
+    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, false, true, Op>), grid, dim3(kBlock), 0, s,
                        static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
   else
-    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, false, Op>), grid, dim3(kBlock), 0, s,
-                       static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
+    hipLaunchKernelGGL((k_apply_small<T, CH, VEC, false, false, Op>), grid, dim3(kBlock), 0,
+                       s, static_cast<const T*>(in), static_cast<T*>(out), p.g, op);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
 }
@@ -905,8 +952,17 @@ extern "C" int qsv_apply_ptm(int dtype, const void* in, void* out, int rank,
                              const int64_t* dims, int k, const int32_t* targets,
                              const int32_t* dims_out_t, const double* ptm_f64,
                              void* workspace, int64_t workspace_bytes, void* stream) {
+  return qsv_apply_ptm_strided(dtype, in, out, rank, dims, nullptr, nullptr, k, targets,
+                               dims_out_t, ptm_f64, workspace, workspace_bytes, stream);
+}
+
+extern "C" int qsv_apply_ptm_strided(int dtype, const void* in, void* out, int rank,
+                                     const int64_t* dims, const int64_t* in_strides,
+                                     const int64_t* out_strides, int k, const int32_t* targets,
+                                     const int32_t* dims_out_t, const double* ptm_f64,
+                                     void* workspace, int64_t workspace_bytes, void* stream) {
   Plan p;
-  const int rc = make_plan(dtype, rank, dims, k, targets, dims_out_t, p);
+  const int rc = make_plan(dtype, rank, dims, in_strides, out_strides, k, targets, dims_out_t, p);
   if (rc) return rc;
   QSV_CHECK_ARG(in && out && ptm_f64, "null pointer argument");
   QSV_CHECK_ARG(in != out, "in and out must not alias");
@@ -935,7 +991,7 @@ extern "C" int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
                                  const int32_t* dims_out_t, const void* ptm_dev,
                                  void* workspace, int64_t workspace_bytes, void* stream) {
   Plan p;
-  const int rc = make_plan(dtype, rank, dims, k, targets, dims_out_t, p);
+  const int rc = make_plan(dtype, rank, dims, nullptr, nullptr, k, targets, dims_out_t, p);
   if (rc) return rc;
   QSV_CHECK_ARG(in && out && ptm_dev, "null pointer argument");
   QSV_CHECK_ARG(in != out, "in and out must not alias");
@@ -963,6 +1019,27 @@ extern "C" int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
 // written exactly once.  Diagnostic only: no GPU work.
 namespace qsv {
 
+// Every logical output entry written exactly once, padding never.
+static int check_coverage(const Plan& p, const std::vector<int>& written, const char* who) {
+  int64_t n_logical = 1;
+  for (int q = 0; q < p.rank; ++q) n_logical *= p.dims_out[q];
+  int64_t total = 0;
+  for (int64_t a = 0; a < p.n_out; ++a) total += written[a];
+  QSV_CHECK_ARG(total == n_logical, "%s: %lld writes for %lld entries", who, (long long)total,
+                (long long)n_logical);
+  std::vector<int64_t> idx(p.rank, 0);
+  for (int64_t e = 0; e < n_logical; ++e) {
+    int64_t r = e, a = 0;
+    for (int q = p.rank - 1; q >= 0; --q) {
+      a += (r % p.dims_out[q]) * p.sout[q];
+      r /= p.dims_out[q];
+    }
+    QSV_CHECK_ARG(written[a] == 1, "%s: output %lld written %d times", who, (long long)a,
+                  written[a]);
+  }
+  return QSV_OK;
+}
+
 static int emulate_direct(const Plan& p, const double* ptm, const double* in, double* out) {
   std::vector<int> written(p.n_out, 0);
   const int vec = p.vec;
@@ -986,10 +1063,7 @@ static int emulate_direct(const Plan& p, const double* ptm, const double* in, do
       }
     }
   }
-  for (int64_t a = 0; a < p.n_out; ++a)
-    QSV_CHECK_ARG(written[a] == 1, "direct: output %lld written %d times", (long long)a,
-                  written[a]);
-  return QSV_OK;
+  return check_coverage(p, written, "direct");
 }
 
 static int emulate_lds(const Plan& p, const LdsPlan& lp, int elem_bytes, const double* ptm,
@@ -1054,10 +1128,7 @@ static int emulate_lds(const Plan& p, const LdsPlan& lp, int elem_bytes, const d
         ++written[g];
       }
   }
-  for (int64_t a = 0; a < p.n_out; ++a)
-    QSV_CHECK_ARG(written[a] == 1, "lds: output %lld written %d times", (long long)a,
-                  written[a]);
-  return QSV_OK;
+  return check_coverage(p, written, "lds");
 }
 
 // Replays k_apply_fused: tile gather, fragment-ordered PTM reads and the
@@ -1119,20 +1190,18 @@ static int emulate_fused(const Plan& p, int elem_bytes, const double* ptm, const
             }
         }
   }
-  for (int64_t a = 0; a < p.n_out; ++a)
-    QSV_CHECK_ARG(written[a] == 1, "fused: output %lld written %d times", (long long)a,
-                  written[a]);
-  return QSV_OK;
+  return check_coverage(p, written, "fused");
 }
 
 }  // namespace qsv
 
 extern "C" int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, int rank,
-                                     const int64_t* dims, int k, const int32_t* targets,
+                                     const int64_t* dims, const int64_t* in_strides,
+                                     const int64_t* out_strides, int k, const int32_t* targets,
                                      const int32_t* dims_out_t, const double* ptm_f64,
                                      int32_t* path_out) {
   Plan p;
-  const int rc = make_plan(dtype, rank, dims, k, targets, dims_out_t, p);
+  const int rc = make_plan(dtype, rank, dims, in_strides, out_strides, k, targets, dims_out_t, p);
   if (rc) return rc;
   QSV_CHECK_ARG(in && out && ptm_f64, "null pointer argument");
   QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24) && p.n_out <= (int64_t(1) << 24),
@@ -1156,7 +1225,7 @@ extern "C" int qsv_apply_ptm_batched(int dtype, const void* in, void* out, int b
                                      const int32_t* dims_out_t, const void* ptm_dev,
                                      int64_t ptm_batch_stride, void* stream) {
   Plan p;
-  const int rc = make_plan(dtype, rank, dims, k, targets, dims_out_t, p);
+  const int rc = make_plan(dtype, rank, dims, nullptr, nullptr, k, targets, dims_out_t, p);
   if (rc) return rc;
   QSV_CHECK_ARG(batch >= 0 && batch <= 65535, "batch %d outside [0, 65535]", batch);
   QSV_CHECK_ARG(in && out && ptm_dev, "null pointer argument");

@@ -147,6 +147,15 @@ extern "C" int qsv_abi_version(void) { return 100; }
 extern "C" int qsv_multitake(int in_dtype, int out_dtype, const void* in, void* out, int rank,
                              const int64_t* in_dims, const int64_t* out_dims,
                              const int32_t* idx_dev, const int32_t* offsets, void* stream) {
+  return qsv_multitake_strided(in_dtype, out_dtype, in, out, rank, in_dims, nullptr, out_dims,
+                               idx_dev, offsets, stream);
+}
+
+extern "C" int qsv_multitake_strided(int in_dtype, int out_dtype, const void* in, void* out,
+                                     int rank, const int64_t* in_dims,
+                                     const int64_t* in_strides, const int64_t* out_dims,
+                                     const int32_t* idx_dev, const int32_t* offsets,
+                                     void* stream) {
   QSV_CHECK_ARG(dtype_size(in_dtype) && dtype_size(out_dtype), "unknown dtype");
   QSV_CHECK_ARG(!(in_dtype == QSV_F64 && out_dtype == QSV_F32),
                 "f64 -> f32 narrowing is not a gather");
@@ -160,6 +169,10 @@ extern "C" int qsv_multitake(int in_dtype, int out_dtype, const void* in, void* 
   int64_t n_out = 1, stride = 1;
   for (int r = rank - 1; r >= 0; --r) {
     QSV_CHECK_ARG(in_dims[r] >= 1 && out_dims[r] >= 0, "bad dims on axis %d", r);
+    if (in_strides) {
+      QSV_CHECK_ARG(in_strides[r] >= stride, "strides of axis %d overlap the axes below", r);
+      stride = in_strides[r];
+    }
     g.in_stride[r] = stride;
     stride *= in_dims[r];
     if (out_dims[r] == 0) n_out = 0;

@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from .. import _native as nat
+from .layout import physical_strides
 from .pauli_vector import PauliVectorBase
 
 __all__ = ['PauliVectorHIP']
@@ -180,20 +181,19 @@ class PauliVectorHIP(PauliVectorBase):
             raise ValueError('`pv` should be Numpy array, torch tensor or None, got type `{}`'
                              .format(type(pv)))
         self._device = _resolve_device(device)
-        n = math.prod(shape)
-        if isinstance(pv, np.ndarray):
-            host = torch.from_numpy(np.ascontiguousarray(pv).reshape(-1))
-            data = host.to(device=self._device, dtype=self._dtype)
-        elif isinstance(pv, torch.Tensor):
-            data = pv.detach().reshape(-1).to(device=self._device, dtype=self._dtype,
-                                               copy=True)
-        else:
-            data = torch.zeros(max(n, 1), dtype=self._dtype, device=self._device)
-            data[0] = 1
-            data = data[:n] if n else data
-        self._store = data            # owning storage of the state
-        self._data = data[:n] if data.numel() != n else data
         self._shape = tuple(shape)
+        self._pstr, extent = physical_strides(self._shape, self._elem_bytes())
+        store = torch.zeros(extent, dtype=self._dtype, device=self._device)
+        view = torch.as_strided(store, self._shape, self._pstr)
+        if isinstance(pv, np.ndarray):
+            view.copy_(torch.from_numpy(np.ascontiguousarray(pv)).to(
+                device=self._device, dtype=self._dtype))
+        elif isinstance(pv, torch.Tensor):
+            view.copy_(pv.detach().to(device=self._device, dtype=self._dtype))
+        else:
+            view[(0,) * len(self._shape)] = 1
+        self._store = store           # owning storage of the state
+        self._data = store            # physical (padded) extent
         self._spare = None            # second buffer (apply_ptm writes there)
 
     # ------------------------------------------------------------ plumbing
@@ -213,11 +213,14 @@ class PauliVectorHIP(PauliVectorBase):
         return self._shape
 
     def to_torch(self):
-        """The device tensor of the expansion (a view, not a copy)."""
-        return self._data.view(self._shape)
+        """The device tensor of the expansion (a strided view, not a copy)."""
+        return torch.as_strided(self._store, self._shape, self._pstr)
 
     def _code(self):
         return nat.dtype_code(self._dtype)
+
+    def _elem_bytes(self):
+        return 8 if self._dtype == torch.float64 else 4
 
     def _take_spare(self, n):
         sp = self._spare
@@ -226,15 +229,16 @@ class PauliVectorHIP(PauliVectorBase):
         self._spare = None
         return sp
 
-    def _swap_in(self, store, n, shape):
+    def _swap_in(self, store, extent, shape, strides):
         self._spare = self._store
         self._store = store
-        self._data = store[:n]
+        self._data = store[:extent]
         self._shape = tuple(shape)
+        self._pstr = tuple(strides)
 
     # ----------------------------------------------------------------- API
     def to_pv(self):
-        return self._data.view(self._shape).cpu().numpy()
+        return self.to_torch().contiguous().cpu().numpy()
 
     def apply_ptm(self, ptm, *qubits):
         """Contract ``ptm`` (shape (out..., in...)) with the listed qudits
@@ -269,22 +273,23 @@ class PauliVectorHIP(PauliVectorBase):
         new_shape = list(self._shape)
         for q, d in zip(qubits, dims_out_t):
             new_shape[q] = d
-        n_out = math.prod(new_shape)
-        out = self._take_spare(n_out)
+        new_str, extent = physical_strides(new_shape, self._elem_bytes())
+        out = self._take_spare(extent)
         code = self._code()
         L = nat.lib()
         ws_bytes = L.qsv_apply_ptm_workspace_bytes(code, k, nat.i32(dims_in_t),
                                                    nat.i32(dims_out_t))
         nat.check(ws_bytes, 'apply_ptm')
         ws = _Scratch.get(self._device, 'ptm', ws_bytes).data_ptr() if ws_bytes else None
-        rc = L.qsv_apply_ptm(
-            code, self._data.data_ptr(), out.data_ptr(), len(self._shape),
-            nat.i64(self._shape), k, nat.i32(qubits), nat.i32(dims_out_t),
-            ptm.ctypes.data_as(nat._c_f64p), ws, ws_bytes, _stream(self._device))
+        rc = L.qsv_apply_ptm_strided(
+            code, self._store.data_ptr(), out.data_ptr(), len(self._shape),
+            nat.i64(self._shape), nat.i64(self._pstr), nat.i64(new_str), k, nat.i32(qubits),
+            nat.i32(dims_out_t), ptm.ctypes.data_as(nat._c_f64p), ws, ws_bytes,
+            _stream(self._device))
         if rc != 0:
             self._spare = out
             nat.check(rc, 'apply_ptm')
-        self._swap_in(out, n_out, new_shape)
+        self._swap_in(out, extent, new_shape, new_str)
 
     # ------------------------------------------------------------ readouts
     def _diag_plan(self):
@@ -313,9 +318,10 @@ class PauliVectorHIP(PauliVectorBase):
         s = _stream(self._device)
         n_take = math.prod(plan.take_dims)
         cur = torch.empty(max(n_take, 1), dtype=torch.float64, device=self._device)
-        rc = L.qsv_multitake(self._code(), nat.QSV_F64, self._data.data_ptr(), cur.data_ptr(),
-                             rank, nat.i64(self._shape), nat.i64(plan.take_dims),
-                             plan.idx.data_ptr(), nat.i32(plan.offsets), s)
+        rc = L.qsv_multitake_strided(self._code(), nat.QSV_F64, self._store.data_ptr(),
+                                     cur.data_ptr(), rank, nat.i64(self._shape),
+                                     nat.i64(self._pstr), nat.i64(plan.take_dims),
+                                     plan.idx.data_ptr(), nat.i32(plan.offsets), s)
         nat.check(rc, 'diagonal')
         cur = cur[:n_take]
         dims = list(plan.take_dims)
@@ -371,10 +377,10 @@ class PauliVectorHIP(PauliVectorBase):
         n_out = math.prod(out_dims)
         out = torch.empty(max(n_out, 1), dtype=torch.float64, device=self._device)
         table = torch.tensor(flat if flat else [0], dtype=torch.int32, device=self._device)
-        rc = nat.lib().qsv_multitake(self._code(), nat.QSV_F64, self._data.data_ptr(),
-                                     out.data_ptr(), len(self._shape), nat.i64(self._shape),
-                                     nat.i64(out_dims), table.data_ptr(), nat.i32(offsets),
-                                     _stream(self._device))
+        rc = nat.lib().qsv_multitake_strided(
+            self._code(), nat.QSV_F64, self._store.data_ptr(), out.data_ptr(),
+            len(self._shape), nat.i64(self._shape), nat.i64(self._pstr), nat.i64(out_dims),
+            table.data_ptr(), nat.i32(offsets), _stream(self._device))
         nat.check(rc, 'multitake')
         res = out[:n_out].view(tuple(out_dims))
         return res.cpu().numpy() if get_data else res
@@ -412,7 +418,7 @@ class PauliVectorHIP(PauliVectorBase):
             self._validate_qubit(q, 'qubit')
         self._check_bases_match()
         traced = [i for i in range(self.n_qubits) if i not in qubits]
-        cur, shape = self._data, list(self._shape)
+        cur, shape, strides = self._store, list(self._shape), self._pstr
         L = nat.lib()
         s = _stream(self._device)
         code = self._code()
@@ -425,14 +431,15 @@ class PauliVectorHIP(PauliVectorBase):
             ws_bytes = nat.check(L.qsv_apply_ptm_workspace_bytes(
                 code, 1, nat.i32([shape[q]]), nat.i32([1])), 'partial_trace')
             ws = _Scratch.get(self._device, 'ptm', ws_bytes).data_ptr() if ws_bytes else None
-            rc = L.qsv_apply_ptm(code, cur.data_ptr(), nxt.data_ptr(), len(shape),
-                                 nat.i64(shape), 1, nat.i32([q]), nat.i32([1]),
-                                 np.ascontiguousarray(t).ctypes.data_as(nat._c_f64p), ws,
-                                 ws_bytes, s)
+            rc = L.qsv_apply_ptm_strided(
+                code, cur.data_ptr(), nxt.data_ptr(), len(shape), nat.i64(shape),
+                nat.i64(strides) if strides is not None else None, None, 1, nat.i32([q]),
+                nat.i32([1]), np.ascontiguousarray(t).ctypes.data_as(nat._c_f64p), ws,
+                ws_bytes, s)
             nat.check(rc, 'partial_trace')
-            cur, shape = nxt[:math.prod(new_shape)], new_shape
+            cur, shape, strides = nxt[:math.prod(new_shape)], new_shape, None
         kept_shape = tuple(self._shape[i] for i in range(self.n_qubits) if i in qubits)
-        data = cur.reshape(kept_shape) if traced else cur.reshape(kept_shape).clone()
+        data = cur.reshape(kept_shape) if traced else self.to_torch()
         return self.__class__([self.bases[q] for q in qubits], data, force=True,
                               dtype=self._dtype, device=self._device)
 
@@ -453,5 +460,5 @@ class PauliVectorHIP(PauliVectorBase):
     def copy(self):
         """Deep copy on the same device (the bases list is new, the basis
         objects are shared, as in numpy.py:119-120)."""
-        return self.__class__(self.bases, self._data.view(self._shape), force=True,
+        return self.__class__(self.bases, self.to_torch(), force=True,
                               dtype=self._dtype, device=self._device)

@@ -52,7 +52,7 @@ def main(argv=None):
         print('done', flush=True)
         return 0
     v = qa.PauliVectorHIP([bases.general(2)] * n, force=True, dtype=dt)
-    v.to_torch().view(-1).copy_(torch.randn(N, dtype=dt, device='cuda'))
+    v.to_torch().copy_(torch.randn((4,) * n, dtype=dt, device='cuda'))
     circuit, _ = noisy_layers(n, 2, seed=1, pairing='brickwork')
     if a.what == 'apply2q':
         gates = [(p, q) for p, q in circuit if len(q) == 2]

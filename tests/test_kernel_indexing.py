@@ -15,22 +15,46 @@ from oracle.pauli_vector import PauliVectorNumpy
 from oracle import bases as ob
 
 
-def emulate(dtype, pv, ptm, targets):
+def _padded(shape, pads):
+    """element strides with pads[q] extra elements per step of axis q"""
+    st, acc = [0] * len(shape), 1
+    for q in reversed(range(len(shape))):
+        st[q] = acc + pads[q]
+        acc = st[q] * shape[q]
+    return st, acc
+
+
+def emulate(dtype, pv, ptm, targets, pads_in=None, pads_out=None):
+    """Run the CPU replay; with pads, the vectors are laid out with padded
+    strides (padding filled with NaN, which must never reach a result)."""
     L = nat.lib()
     k = len(targets)
     dims = pv.shape
     out_shape = list(dims)
     for j, q in enumerate(targets):
         out_shape[q] = ptm.shape[j]
-    out = np.zeros(out_shape)
     path = (nat.ctypes.c_int32 * 1)()
-    pv = np.ascontiguousarray(pv, dtype=np.float64)
     ptm = np.ascontiguousarray(ptm, dtype=np.float64)
-    rc = L.qsv_emulate_apply_ptm(dtype, pv.ctypes.data_as(nat._c_f64p),
-                                 out.ctypes.data_as(nat._c_f64p), len(dims), nat.i64(dims), k,
-                                 nat.i32(targets), nat.i32(ptm.shape[:k]),
-                                 ptm.ctypes.data_as(nat._c_f64p), path)
+    if pads_in is None:
+        pv = np.ascontiguousarray(pv, dtype=np.float64)
+        out = np.zeros(out_shape)
+        rc = L.qsv_emulate_apply_ptm(dtype, pv.ctypes.data_as(nat._c_f64p),
+                                     out.ctypes.data_as(nat._c_f64p), len(dims), nat.i64(dims),
+                                     None, None, k, nat.i32(targets), nat.i32(ptm.shape[:k]),
+                                     ptm.ctypes.data_as(nat._c_f64p), path)
+        nat.check(rc, 'emulate')
+        return out, path[0]
+    si, ni = _padded(dims, pads_in)
+    so, no = _padded(out_shape, pads_out)
+    buf_in = np.full(ni, np.nan)
+    np.lib.stride_tricks.as_strided(buf_in, dims, [8 * x for x in si])[...] = pv
+    buf_out = np.full(no, np.nan)
+    rc = L.qsv_emulate_apply_ptm(dtype, buf_in.ctypes.data_as(nat._c_f64p),
+                                 buf_out.ctypes.data_as(nat._c_f64p), len(dims), nat.i64(dims),
+                                 nat.i64(si), nat.i64(so), k, nat.i32(targets),
+                                 nat.i32(ptm.shape[:k]), ptm.ctypes.data_as(nat._c_f64p), path)
     nat.check(rc, 'emulate')
+    out = np.lib.stride_tricks.as_strided(buf_out, out_shape, [8 * x for x in so]).copy()
     return out, path[0]
 
 
@@ -87,3 +111,24 @@ def test_paths_taken():
     assert emulate(nat.QSV_F64, big, rng.standard_normal([9] * 4), (1, 2))[1] == 3
     assert emulate(nat.QSV_F64, pv, rng.standard_normal([4] * 6), (1, 2, 3))[1] == 4
     assert emulate(nat.QSV_F32, pv, rng.standard_normal([4] * 8), (0, 7, 3, 4))[1] == 4
+
+
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets', [([4] * 6, (0, 1)), ([4] * 6, (5, 4)), ([4] * 6, (2,)),
+                                          ([4] * 6, (0, 5)), ([4] * 5, (1, 2, 3)),
+                                          ([4] * 5, (4, 0, 2, 1)), ([3, 4, 2, 4], (3, 1))])
+def test_emulated_kernels_padded_layout(dims, targets, dtype, rng):
+    """Padded strides on the outer axes (as the backend lays out large
+    vectors): same results, no access to padding (NaN-filled), every logical
+    output written once."""
+    n = len(dims)
+    pads = [4 if q < 3 else 0 for q in range(n)]      # multiples of the vector width
+    pv = rng.standard_normal(dims)
+    din = [dims[t] for t in targets]
+    ptm = rng.standard_normal(din + din)
+    got, path = emulate(dtype, pv, ptm, targets, pads, pads)
+    assert not np.isnan(got).any()
+    assert np.allclose(got, oracle_apply(pv, ptm, targets), rtol=1e-13, atol=1e-13), path
+    odd = [3 if q < 2 else 0 for q in range(n)]         # not a multiple: scalar accesses
+    got, path = emulate(dtype, pv, ptm, targets, odd, [0] * n)
+    assert np.allclose(got, oracle_apply(pv, ptm, targets), rtol=1e-13, atol=1e-13), path
