@@ -166,7 +166,9 @@ int64_t qsv_selftest_fastdiv(int32_t n_divisors, uint64_t seed);
 
 /* CPU replay of qsv_apply_ptm for a vector of at most 2^24 entries: the
  * same planner picks the kernel path the GPU call would take for `dtype`
- * (reported in *path_out: 1 direct, 2 LDS-tiled, 3 table, 4 MFMA dense), then every
+ * (reported in *path_out: 1 direct, 2 LDS-tiled, 3 table, 4 MFMA dense; for
+ * the MFMA path bits 4-7 and 8-11 hold the elements per load and per store
+ * access along the PTM index, bits 12-15 the orbits per lane vector), then every
  * thread's address arithmetic is executed on the host in float64 with each
  * global and LDS access checked against its buffer and every output entry
  * checked to be written exactly once (padding never).  in/out/ptm are host

@@ -871,27 +871,70 @@ static bool use_fused(const Plan& p) {
   return p.din == p.dout && (p.din == 64 || p.din == 256);
 }
 
-// Dense 64 x 64 / 256 x 256 PTMs on MFMA: the workspace holds off_in[D],
-// off_out[D] and the PTM in A-fragment order.
-template <typename T>
-static int run_fused(const Plan& p, const double* ptm_f64, void* workspace, const void* in,
-                     void* out, hipStream_t s) {
+// Plan of the MFMA kernels (index maps in qsv_fused.h).  False when the
+// target offsets are not additive over the bits of the PTM index, which
+// cannot happen for Pauli dimensions (perfect squares whose product is 64 or
+// 256 are powers of 4) but is checked rather than assumed.
+static bool make_fused_plan(const Plan& p, int elem_bytes, const void* in, const void* out,
+                            FusedPlan& fp) {
+  if (!use_fused(p)) return false;
   const int D = p.din;
-  const int64_t need = (2 * int64_t(D)) * 8 + int64_t(D) * D * int64_t(sizeof(T));
-  std::vector<unsigned char> host(need, 0);
-  int64_t* oin = reinterpret_cast<int64_t*>(host.data());
-  int64_t* oout = oin + D;
-  for (int i = 0; i < D; ++i) oin[i] = p.off_in[i];
-  for (int o = 0; o < D; ++o) oout[o] = p.off_out[o];
-  T* pf = reinterpret_cast<T*>(host.data() + 2 * int64_t(D) * 8);
-  for (int o = 0; o < D; ++o)
-    for (int k = 0; k < D; ++k) pf[fused_frag_index(D, o, k)] = T(ptm_f64[int64_t(o) * D + k]);
-  QSV_CHECK_HIP(hipMemcpyAsync(workspace, host.data(), need, hipMemcpyHostToDevice, s));
-  const unsigned char* wb = static_cast<const unsigned char*>(workspace);
-  const uint32_t n_orbits = uint32_t(int64_t(p.g.n_threads) * p.vec);
-  return launch_fused(sizeof(T) == 8 ? QSV_F64 : QSV_F32, D, p.g, n_orbits, in, out,
-                      wb + 2 * int64_t(D) * 8, reinterpret_cast<const int64_t*>(wb),
-                      reinterpret_cast<const int64_t*>(wb) + D, s);
+  for (int k = 0; k < D; ++k) {
+    int64_t si = 0, so = 0;
+    for (int b = 1; b < D; b <<= 1)
+      if (k & b) {
+        si += p.off_in[b];
+        so += p.off_out[b];
+      }
+    if (si != p.off_in[k] || so != p.off_out[k]) return false;
+  }
+  const int VL = 16 / elem_bytes;
+  auto width = [&](const std::vector<int64_t>& off, const int64_t* gs, const void* ptr) {
+    if (reinterpret_cast<uintptr_t>(ptr) % 16) return 1;
+    for (int e = 1; e < VL; ++e)
+      if (off[e] != e) return 1;
+    for (int b = VL; b < D; b <<= 1)
+      if (off[b] % VL) return 1;
+    for (int q = 0; q < p.g.ngroups; ++q)
+      if (gs[q] % VL) return 1;
+    return VL;
+  };
+  fp.D = D;
+  fp.vi = width(p.off_in, p.g.gs_in, in);
+  fp.vo = width(p.off_out, p.g.gs_out, out);
+  // float64 256 x 256: gathering vector stores out of the 64 accumulators
+  // costs an extra 128 VGPRs (one wave per SIMD); scalar stores keep two
+  if (D == 256 && elem_bytes == 8) fp.vo = 1;
+  fp.n_orbits = uint32_t(int64_t(p.g.n_threads) * p.vec);
+  // 64 x 64 with a unit-stride innermost bystander axis: lanes own p.vec
+  // consecutive orbits (the planner's vector width already checks that every
+  // stride and offset keeps such vectors aligned)
+  fp.vn = 1;
+  if (D == 64 && fp.vi == 1 && fp.vo == 1 && p.vec > 1 &&
+      reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0)
+    fp.vn = p.vec;
+  memset(&fp.offs, 0, sizeof(fp.offs));
+  const int KS = D / 4;
+  for (int u = 0; u < KS / fp.vi; ++u) fp.offs.in_u[u] = p.off_in[fused_k(fp.vi, fp.vi * u, 0)];
+  for (int v = 0; v < KS / fp.vo; ++v) fp.offs.out_u[v] = p.off_out[fused_o(fp.vo, fp.vo * v, 0)];
+  for (int j = 0; j < 4; ++j) {
+    fp.offs.in_l[j] = p.off_in[fused_k(fp.vi, 0, j)];
+    fp.offs.out_l[j] = p.off_out[fused_o(fp.vo, 0, j)];
+  }
+  return true;
+}
+
+// Dense 64 x 64 / 256 x 256 PTMs on MFMA: the workspace holds the PTM in
+// fragment order.
+template <typename T>
+static int run_fused(const Plan& p, const FusedPlan& fp, const double* ptm_f64, void* workspace,
+                     const void* in, void* out, hipStream_t s) {
+  const int D = fp.D;
+  std::vector<T> host(size_t(D) * D);
+  fused_fragments<T>(D, int(sizeof(T)), fp.vi, fp.vo, ptm_f64, host.data());
+  QSV_CHECK_HIP(hipMemcpyAsync(workspace, host.data(), host.size() * sizeof(T),
+                               hipMemcpyHostToDevice, s));
+  return launch_fused(sizeof(T) == 8 ? QSV_F64 : QSV_F32, fp, p.g, p.nt, in, out, workspace, s);
 }
 
 template <typename T>
@@ -979,9 +1022,10 @@ extern "C" int qsv_apply_ptm_strided(int dtype, const void* in, void* out, int r
   QSV_CHECK_ARG(workspace && workspace_bytes >= need,
                 "a %d x %d PTM needs a workspace of %lld bytes", p.dout, p.din,
                 (long long)need);
-  if (use_fused(p))
-    return dtype == QSV_F64 ? run_fused<double>(p, ptm_f64, workspace, in, out, s)
-                            : run_fused<float>(p, ptm_f64, workspace, in, out, s);
+  FusedPlan fp;
+  if (make_fused_plan(p, dtype_size(dtype), in, out, fp))
+    return dtype == QSV_F64 ? run_fused<double>(p, fp, ptm_f64, workspace, in, out, s)
+                            : run_fused<float>(p, fp, ptm_f64, workspace, in, out, s);
   return dtype == QSV_F64 ? run_host_table<double>(p, ptm_f64, workspace, in, out, s)
                           : run_host_table<float>(p, ptm_f64, workspace, in, out, s);
 }
@@ -1131,64 +1175,109 @@ static int emulate_lds(const Plan& p, const LdsPlan& lp, int elem_bytes, const d
   return check_coverage(p, written, "lds");
 }
 
-// Replays k_apply_fused: tile gather, fragment-ordered PTM reads and the
-// accumulator-to-HBM mapping of every lane.
+// Replays k_fused64 / k_fused256 lane by lane: the B-fragment loads (with
+// their vector alignment and, for orbit vectors, the consecutive-orbit
+// assumption), the fragment-ordered PTM reads, the MFMA contraction in
+// hardware index order and the accumulator stores.
 static int emulate_fused(const Plan& p, int elem_bytes, const double* ptm, const double* in,
                          double* out) {
-  const int D = p.din, G = fused_cols(D), MB = D / 64, NB = G / 16, KS = D / 4;
-  const uint32_t n_orbits = uint32_t(int64_t(p.g.n_threads) * p.vec);
+  FusedPlan fp;
+  QSV_CHECK_ARG(make_fused_plan(p, elem_bytes, nullptr, nullptr, fp),
+                "fused: offsets not additive");
+  const int D = fp.D, MB = D / 16, KS = D / 4;
+  const int vi = fp.vi, vo = fp.vo, vn = fp.vn;
   std::vector<double> pf(int64_t(D) * D);
-  for (int o = 0; o < D; ++o)
-    for (int k = 0; k < D; ++k) {
-      const int64_t f = fused_frag_index(D, o, k);
-      QSV_CHECK_ARG(f >= 0 && f < int64_t(D) * D, "fused: fragment index out of range");
-      pf[f] = ptm[int64_t(o) * D + k];
-    }
-  std::vector<int> written(p.n_out, 0);
-  const uint32_t tiles = (n_orbits + G - 1) / G;
-  std::vector<double> Bs(size_t(D) * (G + 4));
-  for (uint32_t t = 0; t < tiles; ++t) {
-    const uint32_t g0 = t * G;
-    for (int tid = 0; tid < kBlock; ++tid) {
-      const int c = tid % G;
-      const uint32_t gi = g0 + c;
-      int64_t bin = 0, bout = 0;
-      if (gi < n_orbits) orbit_base(p.g, gi, bin, bout);
-      for (int k = tid / G; k < D; k += kBlock / G) {
-        double x = 0;
-        if (gi < n_orbits) {
-          const int64_t a = bin + p.off_in[k];
-          QSV_CHECK_ARG(a >= 0 && a < p.n_in, "fused: load %lld out of bounds", (long long)a);
-          x = in[a];
+  fused_fragments<double>(D, elem_bytes, vi, vo, ptm, pf.data());
+  {
+    // the fragment table must hold every PTM entry exactly once
+    std::vector<int> used(int64_t(D) * D, 0);
+    for (int s = 0; s < KS; ++s)
+      for (int mb = 0; mb < MB; ++mb)
+        for (int lane = 0; lane < 64; ++lane) {
+          int j, r;
+          mfma_row_owner(elem_bytes, lane & 15, j, r);
+          const int o = fused_o(vo, 4 * mb + r, j), k = fused_k(vi, s, lane >> 4);
+          QSV_CHECK_ARG(o >= 0 && o < D && k >= 0 && k < D, "fused: map out of range");
+          ++used[int64_t(o) * D + k];
         }
-        Bs[size_t(k) * (G + 4) + c] = x;
+    for (int64_t e = 0; e < int64_t(D) * D; ++e)
+      QSV_CHECK_ARG(used[e] == 1, "fused: PTM entry %lld used %d times", (long long)e, used[e]);
+  }
+  const int NB = fused_nb(elem_bytes, D, vn);
+  const uint32_t cols = 16 * NB, n_orbits = fp.n_orbits;
+  QSV_CHECK_ARG(n_orbits % vn == 0, "fused: orbit count not a multiple of the vector");
+  const uint32_t ntiles = (n_orbits + cols - 1) / cols;
+  std::vector<int> written(p.n_out, 0);
+  std::vector<double> B(size_t(64) * NB * KS);
+  // orbit of (tile, nb, n) and its base addresses; vn > 1 derives the bases
+  // of a lane's vn orbits from the first one, as the kernel does
+  auto bases = [&](uint32_t t, int nb, int n, bool clamp, int64_t& bin, int64_t& bout,
+                   uint32_t& m) -> int {
+    const uint32_t m0 = fused64_orbit_of(NB, vn, t, 0, n);
+    m = fused64_orbit_of(NB, vn, t, nb, n);
+    if (vn > 1) {
+      const uint32_t mb0 = clamp && m0 >= n_orbits ? n_orbits - vn : m0;
+      int64_t b0i, b0o, bi, bo;
+      orbit_base(p.g, mb0, b0i, b0o);
+      orbit_base(p.g, mb0 + nb, bi, bo);
+      QSV_CHECK_ARG(bi == b0i + nb && bo == b0o + nb, "fused: orbits of a vector not adjacent");
+      QSV_CHECK_ARG(b0i % vn == 0 && b0o % vn == 0, "fused: orbit vector misaligned");
+      bin = bi;
+      bout = bo;
+    } else {
+      orbit_base(p.g, clamp && m >= n_orbits ? n_orbits - 1 : m, bin, bout);
+    }
+    return QSV_OK;
+  };
+  for (uint32_t t = 0; t < ntiles; ++t) {
+    for (int lane = 0; lane < 64; ++lane) {
+      const int n = lane & 15, j = lane >> 4;
+      for (int nb = 0; nb < NB; ++nb) {
+        int64_t bin, bout;
+        uint32_t m;
+        if (bases(t, nb, n, true, bin, bout, m)) return QSV_EINVAL;
+        for (int u = 0; u < KS / vi; ++u) {
+          const int64_t a0 = bin + fp.offs.in_l[j] + fp.offs.in_u[u];
+          QSV_CHECK_ARG((a0 - (vn > 1 ? nb : 0)) % std::max(vi, vn) == 0,
+                        "fused: misaligned vector load");
+          for (int e = 0; e < vi; ++e) {
+            QSV_CHECK_ARG(a0 + e >= 0 && a0 + e < p.n_in, "fused: load %lld out of bounds",
+                          (long long)(a0 + e));
+            B[(size_t(lane) * NB + nb) * KS + u * vi + e] = in[a0 + e];
+          }
+        }
       }
     }
-    for (int w = 0; w < 4; ++w)
-      for (int lane = 0; lane < 64; ++lane)
-        for (int nb = 0; nb < NB; ++nb) {
-          const uint32_t gi = g0 + 16 * nb + (lane & 15);
-          if (gi >= n_orbits) continue;
-          int64_t bin, bout;
-          orbit_base(p.g, gi, bin, bout);
-          for (int mb = 0; mb < MB; ++mb)
-            for (int r = 0; r < 4; ++r) {
-              const int row = w * (D / 4) + 16 * mb + mfma_acc_row(elem_bytes, lane, r);
-              // the accumulator element = sum over k-steps of A(row, k) B(k, col)
-              double acc = 0;
-              for (int sk = 0; sk < KS; ++sk)
-                for (int kk = 0; kk < 4; ++kk) {
-                  const int k = 4 * sk + kk;
-                  acc += pf[fused_frag_index(D, row, k)] *
-                         Bs[size_t(k) * (G + 4) + 16 * nb + (lane & 15)];
-                }
-              const int64_t a = bout + p.off_out[row];
-              QSV_CHECK_ARG(a >= 0 && a < p.n_out, "fused: store %lld out of bounds",
-                            (long long)a);
-              out[a] = acc;
-              ++written[a];
-            }
+    for (int lane = 0; lane < 64; ++lane) {
+      const int n = lane & 15, j = lane >> 4;
+      for (int nb = 0; nb < NB; ++nb) {
+        int64_t bin, bout;
+        uint32_t m;
+        if (bases(t, nb, n, false, bin, bout, m)) return QSV_EINVAL;
+        if (m >= n_orbits) continue;
+        for (int v = 0; v < KS / vo; ++v) {
+          const int64_t a0 = bout + fp.offs.out_l[j] + fp.offs.out_u[v];
+          QSV_CHECK_ARG((a0 - (vn > 1 ? nb : 0)) % std::max(vo, vn) == 0,
+                        "fused: misaligned vector store");
+          for (int e = 0; e < vo; ++e) {
+            const int q = v * vo + e, mb = q >> 2, r = q & 3;
+            const int row16 = elem_bytes == 8 ? j + 4 * r : 4 * j + r;
+            // D[row][n] = sum over k-steps s and k-lanes jk of A[row][4 s + jk] B[4 s + jk][n]
+            double acc = 0;
+            for (int s = 0; s < KS; ++s)
+              for (int jk = 0; jk < 4; ++jk) {
+                const int64_t f = fused_frag_slot(D, s, mb, row16 + 16 * jk);
+                QSV_CHECK_ARG(f >= 0 && f < int64_t(D) * D, "fused: fragment slot out of range");
+                acc += pf[f] * B[(size_t(n + 16 * jk) * NB + nb) * KS + s];
+              }
+            const int64_t a = a0 + e;
+            QSV_CHECK_ARG(a >= 0 && a < p.n_out, "fused: store %lld out of bounds", (long long)a);
+            out[a] = acc;
+            ++written[a];
+          }
         }
+      }
+    }
   }
   return check_coverage(p, written, "fused");
 }
@@ -1212,8 +1301,9 @@ extern "C" int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, i
     if (path_out) *path_out = 2;
     return emulate_lds(p, lp, es, ptm_f64, in, out);
   }
-  if (!fits_inline(p.din, p.dout) && use_fused(p)) {
-    if (path_out) *path_out = 4;
+  FusedPlan fp;
+  if (!fits_inline(p.din, p.dout) && make_fused_plan(p, es, nullptr, nullptr, fp)) {
+    if (path_out) *path_out = 4 | (fp.vi << 4) | (fp.vo << 8) | (fp.vn << 12);
     return emulate_fused(p, es, ptm_f64, in, out);
   }
   if (path_out) *path_out = fits_inline(p.din, p.dout) ? 1 : 3;

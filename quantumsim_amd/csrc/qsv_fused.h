@@ -1,28 +1,105 @@
 // Dense k-qudit PTM application on MFMA (64 x 64 and 256 x 256 PTMs).
+//
+//   out[o, g] = sum_k P[o, k] in[k, g],   g over the N / D orbits
+//
+// Every wave owns columns (orbits) 16 at a time and computes all D output
+// rows of them with v_mfma_{f64,f32}_16x16x4.  The wave loads its B operand
+// (the orbit's D input entries) straight from HBM into the B-fragment
+// registers and stores the accumulators straight back: no LDS round trip
+// for the state.  The PTM (A operand) sits in LDS in fragment order.
+//
+// Within one MFMA k-step lane (n, j) = (lane % 16, lane / 16) supplies
+// B[4 s + j][n] and receives accumulator rows that depend on j.  Which PTM
+// column ("k") and row ("o") those hardware indices stand for is free: the
+// contraction does not care about the order of k, and the rows only decide
+// where a result is stored.  The maps
+//
+//   k = fused_k(VI, s, j) = 4 VI (s / VI) + VI j + s % VI
+//   o = fused_o(VO, q, j) = 4 VO (q / VO) + VO j + q % VO   (q = 4 mb + r)
+//
+// give every lane VI consecutive k (VO consecutive o) per access, so a
+// 16-byte vector load (store) covers them when the lowest PTM index has unit
+// stride in memory (the innermost axis is a target).  Otherwise VI = VO = 1
+// and neighbouring lanes read neighbouring orbits.
 #pragma once
 
 #include "qsv_geom.h"
 
 namespace qsv {
 
-// Columns (orbits) per tile of the MFMA kernel for a D x D PTM.
-constexpr int fused_cols(int D) { return 64; }
+struct FusedOffs {
+  int64_t in_u[64];   // off_in[fused_k(VI, VI u, 0)],  u < (D / 4) / VI
+  int64_t out_u[64];  // off_out[fused_o(VO, VO v, 0)], v < (D / 4) / VO
+  int64_t in_l[4];    // off_in[fused_k(VI, 0, j)]
+  int64_t out_l[4];   // off_out[fused_o(VO, 0, j)]
+};
 
-// Row of the 16 x 16 accumulator held in slot r by `lane` of
-// v_mfma_{f64,f32}_16x16x4 (layouts measured on gfx950: float64 interleaves
-// rows by 4, float32 keeps 4 consecutive rows per lane group).
-__host__ __device__ inline int mfma_acc_row(int elem_bytes, int lane, int r) {
-  return elem_bytes == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+struct FusedPlan {
+  int D;             // 64 or 256
+  int vi, vo;        // elements per load / store access along k / o
+  int vn;            // 64 x 64 only: consecutive orbits per lane (vi = vo = 1)
+  uint32_t n_orbits;
+  FusedOffs offs;
+};
+
+__host__ __device__ inline int fused_k(int vi, int s, int j) {
+  return 4 * vi * (s / vi) + vi * j + s % vi;
+}
+__host__ __device__ inline int fused_o(int vo, int q, int j) {
+  return 4 * vo * (q / vo) + vo * j + q % vo;
 }
 
-// Index of P[row][k] in the fragment-ordered PTM: A operand of m-block
-// row / 16, k-step k / 4, lane (row % 16) + 16 (k % 4).
-__host__ __device__ inline int64_t fused_frag_index(int D, int row, int k) {
-  return (int64_t(row >> 4) * (D >> 2) + (k >> 2)) * 64 + (row & 15) + 16 * (k & 3);
+// Accumulator layout of v_mfma_{f64,f32}_16x16x4 (measured on gfx950): slot
+// r of lane group j holds row j + 4 r (float64) or 4 j + r (float32) of the
+// 16 x 16 block.  Inverse: the (j, r) that hold row m.
+__host__ __device__ inline void mfma_row_owner(int elem_bytes, int m, int& j, int& r) {
+  if (elem_bytes == 8) {
+    j = m & 3;
+    r = m >> 2;
+  } else {
+    j = m >> 2;
+    r = m & 3;
+  }
 }
 
-int launch_fused(int dtype, int D, const Geom& g, uint32_t n_orbits, const void* in, void* out,
-                 const void* ptm_frag, const int64_t* off_in, const int64_t* off_out,
-                 hipStream_t s);
+// Index of the PTM's A fragment for k-step s, 16-row block mb, lane
+// (m + 16 j') in the fragment-ordered table: k-step major, so that a run of
+// k-steps is one contiguous chunk.
+__host__ __device__ inline int64_t fused_frag_slot(int D, int s, int mb, int lane) {
+  return (int64_t(s) * (D / 16) + mb) * 64 + lane;
+}
+
+// Fills pf[D * D] (fragment order) from the row-major float64 PTM.
+template <typename T>
+inline void fused_fragments(int D, int elem_bytes, int vi, int vo, const double* ptm, T* pf) {
+  const int MB = D / 16, KS = D / 4;
+  for (int s = 0; s < KS; ++s)
+    for (int mb = 0; mb < MB; ++mb)
+      for (int lane = 0; lane < 64; ++lane) {
+        const int m = lane & 15, jk = lane >> 4;
+        int j, r;
+        mfma_row_owner(elem_bytes, m, j, r);
+        const int o = fused_o(vo, 4 * mb + r, j);
+        const int k = fused_k(vi, s, jk);
+        pf[fused_frag_slot(D, s, mb, lane)] = T(ptm[int64_t(o) * D + k]);
+      }
+}
+
+// k-steps per LDS chunk of the 256 x 256 kernel (4096 fragment entries)
+constexpr int kFusedChunkSteps = 4;
+
+int launch_fused(int dtype, const FusedPlan& fp, const Geom& g, bool nt, const void* in,
+                 void* out, const void* ptm_frag, hipStream_t s);
+
+// Column blocks per wave tile: the 64 x 64 kernel gives each wave tiles of
+// 16 * nb orbits, the 256 x 256 kernel each block tiles of 4 x 16 orbits
+// (one column block per wave).
+int fused_nb(int elem_bytes, int D, int vn);
+
+// Orbit handled by lane n in column block nb of tile `tile` (64 x 64 kernel).
+__host__ __device__ inline uint32_t fused64_orbit_of(int nbs, int vn, uint32_t tile, int nb,
+                                                     int n) {
+  return vn > 1 ? tile * (16 * nbs) + vn * n + nb : tile * (16 * nbs) + 16 * nb + n;
+}
 
 }  // namespace qsv

@@ -1,15 +1,19 @@
 // Dense D x D PTM application (D = 64: three qubits, D = 256: four qubits)
-// on the gfx950 matrix cores.
+// on the gfx950 matrix cores.  Index maps and the fragment table are
+// described in qsv_fused.h.
 //
-//   out[o, g] = sum_k P[o, k] in[k, g],   g over the N / D orbits
+// k_fused64: the 64 x 64 PTM (32 KB in float64) is staged in LDS once per
+// block; after that every wave runs on its own: it walks tiles of 16 NB
+// orbits, loading the next tile's B fragments while the MFMAs of the
+// current one run (two register buffers), then stores the accumulators.
+// Memory-bound: 2 D element accesses per orbit against 2 D^2 flops.
 //
-// A block owns a tile of G = 64 orbits.  The D x G operand tile is gathered
-// into LDS (lanes walk consecutive orbits, so each row of the tile is one
-// coalesced run when the innermost axis is a bystander).  Each of the four
-// waves computes D / 4 output rows for all G columns with
-// v_mfma_{f64,f32}_16x16x4: the PTM arrives pre-arranged in A-fragment order
-// (one contiguous 64-lane load per k-step, L2 resident), the tile supplies
-// the B fragments, and the accumulators are stored straight to HBM.
+// k_fused256: the 256 x 256 PTM (512 KB in float64) does not fit in LDS; a
+// block of four waves takes 64 orbits (16 per wave, accumulators for all
+// 256 rows in registers) and streams the PTM through LDS in chunks of four
+// k-steps (4096 entries), double-buffered: the next chunk and the next B
+// fragments are loaded while the current chunk's 64 MFMAs per wave run.
+// MFMA-bound.
 #include "qsv_fused.h"
 
 namespace qsv {
@@ -31,104 +35,301 @@ struct Mfma<float> {
   }
 };
 
-template <typename T, int D>
-__global__ void __launch_bounds__(kBlock)
-    k_apply_fused(const T* __restrict__ in, T* __restrict__ out, const Geom g,
-                  const uint32_t n_orbits, const T* __restrict__ pf,
-                  const int64_t* __restrict__ off_in, const int64_t* __restrict__ off_out) {
-  constexpr int G = fused_cols(D);
-  constexpr int LD = G + 4;      // padded LDS row
-  constexpr int MB = D / 64;     // 16-row blocks per wave
-  constexpr int NB = G / 16;     // 16-column blocks
-  constexpr int KS = D / 4;      // k-steps
-  typedef typename Mfma<T>::acc Acc;
-  extern __shared__ __align__(16) unsigned char qsv_fused_smem[];
-  T* Bs = reinterpret_cast<T*>(qsv_fused_smem);
+int fused_nb(int elem_bytes, int D, int vn) {
+  if (D != 64) return 1;
+  return vn > 1 ? vn : (elem_bytes == 4 ? 2 : 1);
+}
 
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const uint32_t g0 = blockIdx.x * uint32_t(G);
-
-  // gather the D x G operand tile
-  {
-    const int c = tid % G;
-    const uint32_t gi = g0 + c;
-    const bool ok = gi < n_orbits;
-    int64_t bin = 0, bout = 0;
-    if (ok) orbit_base(g, gi, bin, bout);
-    for (int k = tid / G; k < D; k += kBlock / G) Bs[k * LD + c] = ok ? in[bin + off_in[k]] : T(0);
+template <typename T, int V, bool NT>
+__device__ __forceinline__ void load_vec(const T* p, T* dst) {
+  typedef typename VT<T, V>::t Vec;
+  const Vec v = gload<NT>(reinterpret_cast<const Vec*>(p));
+  if constexpr (V == 1) {
+    dst[0] = v;
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; ++e) dst[e] = v[e];
   }
+}
+
+template <typename T, int V, bool NT>
+__device__ __forceinline__ void store_vec(T* p, const T* vals) {
+  typedef typename VT<T, V>::t Vec;
+  Vec v;
+  if constexpr (V == 1) {
+    v = vals[0];
+  } else {
+#pragma unroll
+    for (int e = 0; e < V; ++e) v[e] = vals[e];
+  }
+  gstore<NT>(reinterpret_cast<Vec*>(p), v);
+}
+
+// Memory accesses of the loops below are unconditional (the partial tail
+// tile is handled separately with guards): a global access under a branch
+// makes the compiler's wait-count bookkeeping conservative at the join.
+//
+// Column (orbit) of lane n in column block nb of a tile:
+//   VN = 1: tile * 16 NB + 16 nb + n
+//   VN > 1: tile * 16 VN + VN n + nb   (NB = VN: lane n owns VN consecutive
+//           orbits, loaded and stored as one 16-byte vector when the
+//           innermost axis is a bystander with unit stride)
+
+template <typename T, int VI, int VO, int VN, bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_fused64(const T* __restrict__ in, T* __restrict__ out, const Geom g,
+              const uint32_t n_orbits, const T* __restrict__ pf, const FusedOffs fo) {
+  constexpr int MB = 4, KS = 16;
+  constexpr int NB = VN > 1 ? VN : (sizeof(T) == 4 ? 2 : 1);
+  constexpr int NL = VN > 1 ? 1 : NB;  // orbit bases per lane
+  constexpr uint32_t COLS = 16 * NB;
+  typedef typename Mfma<T>::acc Acc;
+  typedef typename VT<T, VN>::t VNv;
+  __shared__ T As[KS * MB * 64];
+  for (int i = threadIdx.x; i < KS * MB * 64; i += kBlock) As[i] = pf[i];
   __syncthreads();
 
+  const int lane = threadIdx.x & 63, n = lane & 15, j = lane >> 4;
+  const uint32_t nw = gridDim.x * (kBlock / 64);
+  const uint32_t wid = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint32_t full = n_orbits / COLS;
+  const int64_t li = fo.in_l[j], lo = fo.out_l[j];
+
+  T b[NB][KS];
   Acc acc[MB][NB];
+  auto load = [&](uint32_t tile) {
 #pragma unroll
-  for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = Acc(0);
-
-  const T* pw = pf + int64_t(w * MB) * KS * 64 + lane;
-#pragma unroll 4
-  for (int s = 0; s < KS; ++s) {
-    T b[NB];
-    const int krow = (4 * s + (lane >> 4)) * LD + (lane & 15);
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) b[nb] = Bs[krow + 16 * nb];
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) {
-      const T a = pw[(int64_t(mb) * KS + s) * 64];
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = Mfma<T>::op(a, b[nb], acc[mb][nb]);
-    }
-  }
-
-  // accumulators -> HBM
-#pragma unroll
-  for (int nb = 0; nb < NB; ++nb) {
-    const uint32_t gi = g0 + 16 * nb + (lane & 15);
-    if (gi < n_orbits) {
+    for (int l = 0; l < NL; ++l) {
+      uint32_t m = fused64_orbit_of(NB, VN, tile, l, n);
+      if (m >= n_orbits) m = n_orbits - VN;  // tail tile only: loaded, never stored
       int64_t bin, bout;
-      orbit_base(g, gi, bin, bout);
+      orbit_base(g, m, bin, bout);
+      const T* p = in + bin + li;
+      if constexpr (VN > 1) {
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb)
+        for (int s = 0; s < KS; ++s) {
+          const VNv v = gload<NT>(reinterpret_cast<const VNv*>(p + fo.in_u[s]));
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = w * (D / 4) + 16 * mb + mfma_acc_row(sizeof(T), lane, r);
-          out[bout + off_out[row]] = acc[mb][nb][r];
+          for (int e = 0; e < VN; ++e) b[e][s] = v[e];
         }
+      } else {
+#pragma unroll
+        for (int u = 0; u < KS / VI; ++u) load_vec<T, VI, NT>(p + fo.in_u[u], &b[l][u * VI]);
+      }
+    }
+  };
+  auto compute = [&]() {
+    // the LDS offset is made opaque so the compiler re-reads the PTM from
+    // LDS every tile instead of pinning all 64 fragments in VGPRs (which
+    // would halve the waves per SIMD)
+    int abase = lane;
+    asm volatile("" : "+v"(abase));
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = Acc(0);
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const T a = As[(s * MB + mb) * 64 + abase];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = Mfma<T>::op(a, b[nb][s], acc[mb][nb]);
+      }
+  };
+  auto store = [&](int l, uint32_t m) {
+    int64_t bin, bout;
+    orbit_base(g, m, bin, bout);
+    T* p = out + bout + lo;
+    if constexpr (VN > 1) {
+#pragma unroll
+      for (int q = 0; q < 4 * MB; ++q) {
+        VNv v;
+#pragma unroll
+        for (int e = 0; e < VN; ++e) v[e] = acc[q >> 2][e][q & 3];
+        gstore<NT>(reinterpret_cast<VNv*>(p + fo.out_u[q]), v);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4 * MB / VO; ++v) {
+        T vals[VO];
+#pragma unroll
+        for (int e = 0; e < VO; ++e) {
+          const int q = v * VO + e;
+          vals[e] = acc[q >> 2][l][q & 3];
+        }
+        store_vec<T, VO, NT>(p + fo.out_u[v], vals);
+      }
+    }
+  };
+
+  for (uint32_t t = wid; t < full; t += nw) {
+    load(t);
+    compute();
+#pragma unroll
+    for (int l = 0; l < NL; ++l) store(l, fused64_orbit_of(NB, VN, t, l, n));
+  }
+  if (full * COLS < n_orbits && wid == full % nw) {
+    load(full);
+    compute();
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const uint32_t m = fused64_orbit_of(NB, VN, full, l, n);
+      if (m < n_orbits) store(l, m);
     }
   }
 }
 
-template <typename T, int D>
-static int launch_t(const Geom& g, uint32_t n_orbits, const void* in, void* out, const void* pf,
-                    const int64_t* off_in, const int64_t* off_out, hipStream_t s) {
-  constexpr int G = fused_cols(D);
-  const size_t lds = size_t(D) * (G + 4) * sizeof(T);
-  static bool attr_set = false;
-  if (!attr_set) {
-    QSV_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_fused<T, D>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    attr_set = true;
+template <typename T, int VI, int VO, bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_fused256(const T* __restrict__ in, T* __restrict__ out, const Geom g,
+               const uint32_t n_orbits, const T* __restrict__ pf, const FusedOffs fo) {
+  constexpr int MB = 16, KS = 64, CS = kFusedChunkSteps, NCH = KS / CS;
+  constexpr int CHUNK = CS * MB * 64;
+  constexpr int PER = CHUNK / kBlock;
+  constexpr uint32_t COLS = 16 * (kBlock / 64);
+  static_assert(CS % VI == 0, "chunk must hold whole vectors");
+  typedef typename Mfma<T>::acc Acc;
+  __shared__ T As[2][CHUNK];
+
+  const int tid = threadIdx.x, lane = tid & 63, n = lane & 15, j = lane >> 4, w = tid >> 6;
+  const uint32_t ntiles = (n_orbits + COLS - 1) / COLS;
+  const int64_t li = fo.in_l[j], lo = fo.out_l[j];
+
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t m = tile * COLS + 16 * w + n;
+    int64_t bin, bout;
+    orbit_base(g, m < n_orbits ? m : n_orbits - 1, bin, bout);
+    const T* pin = in + bin + li;
+
+    Acc acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) acc[mb] = Acc(0);
+    T ar[PER], bc[CS], bn[CS];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) ar[i] = pf[i * kBlock + tid];
+#pragma unroll
+    for (int uu = 0; uu < CS / VI; ++uu) load_vec<T, VI, NT>(pin + fo.in_u[uu], &bn[uu * VI]);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) As[0][i * kBlock + tid] = ar[i];
+    __syncthreads();
+
+#pragma unroll 1
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+      for (int ss = 0; ss < CS; ++ss) bc[ss] = bn[ss];
+      // next chunk (the last iteration re-reads its own: no branch)
+      const int cn = c + 1 < NCH ? c + 1 : c;
+      const T* src = pf + int64_t(cn) * CHUNK + tid;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) ar[i] = src[i * kBlock];
+#pragma unroll
+      for (int uu = 0; uu < CS / VI; ++uu)
+        load_vec<T, VI, NT>(pin + fo.in_u[cn * (CS / VI) + uu], &bn[uu * VI]);
+      const T* A = As[c & 1];
+#pragma unroll
+      for (int ss = 0; ss < CS; ++ss)
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          acc[mb] = Mfma<T>::op(A[(ss * MB + mb) * 64 + lane], bc[ss], acc[mb]);
+      // the other buffer: nobody reads it in this iteration
+      T* dst = As[(c + 1) & 1] + tid;
+#pragma unroll
+      for (int i = 0; i < PER; ++i) dst[i * kBlock] = ar[i];
+      __syncthreads();
+    }
+
+    if (m < n_orbits) {
+      T* p = out + bout + lo;
+#pragma unroll
+      for (int v = 0; v < 4 * MB / VO; ++v) {
+        T vals[VO];
+#pragma unroll
+        for (int e = 0; e < VO; ++e) {
+          const int q = v * VO + e;
+          vals[e] = acc[q >> 2][q & 3];
+        }
+        store_vec<T, VO, NT>(p + fo.out_u[v], vals);
+      }
+    }
   }
-  const uint32_t tiles = (n_orbits + G - 1) / G;
-  if (tiles == 0) return QSV_OK;
-  hipLaunchKernelGGL((k_apply_fused<T, D>), dim3(tiles), dim3(kBlock), lds, s,
-                     static_cast<const T*>(in), static_cast<T*>(out), g, n_orbits,
-                     static_cast<const T*>(pf), off_in, off_out);
+}
+
+// Blocks that fit on the device at once (occupancy x compute units).
+static uint32_t resident_blocks(const void* kern) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  (void)hipGetLastError();
+  return uint32_t(per_cu) * uint32_t(cus);
+}
+
+template <typename T, int VI, int VO, int VN, bool NT>
+static int launch64(const FusedPlan& fp, const Geom& g, const void* in, void* out, const void* pf,
+                    hipStream_t s) {
+  constexpr int NB = VN > 1 ? VN : (sizeof(T) == 4 ? 2 : 1);
+  const auto kern = k_fused64<T, VI, VO, VN, NT>;
+  static const uint32_t cap = resident_blocks(reinterpret_cast<const void*>(kern));
+  const uint32_t ntiles = (fp.n_orbits + 16 * NB - 1) / (16 * NB);
+  const uint32_t want = (ntiles + kBlock / 64 - 1) / (kBlock / 64);
+  hipLaunchKernelGGL(kern, dim3(want < cap ? want : cap), dim3(kBlock), 0, s,
+                     static_cast<const T*>(in), static_cast<T*>(out), g, fp.n_orbits,
+                     static_cast<const T*>(pf), fp.offs);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
 }
 
-int launch_fused(int dtype, int D, const Geom& g, uint32_t n_orbits, const void* in, void* out,
-                 const void* pf, const int64_t* off_in, const int64_t* off_out, hipStream_t s) {
-  if (dtype == QSV_F64) {
-    if (D == 64) return launch_t<double, 64>(g, n_orbits, in, out, pf, off_in, off_out, s);
-    if (D == 256) return launch_t<double, 256>(g, n_orbits, in, out, pf, off_in, off_out, s);
-  } else {
-    if (D == 64) return launch_t<float, 64>(g, n_orbits, in, out, pf, off_in, off_out, s);
-    if (D == 256) return launch_t<float, 256>(g, n_orbits, in, out, pf, off_in, off_out, s);
+template <typename T, int VI, int VO, bool NT>
+static int launch256(const FusedPlan& fp, const Geom& g, const void* in, void* out,
+                     const void* pf, hipStream_t s) {
+  const auto kern = k_fused256<T, VI, VO, NT>;
+  static const uint32_t cap = resident_blocks(reinterpret_cast<const void*>(kern));
+  const uint32_t want = (fp.n_orbits + 63) / 64;
+  hipLaunchKernelGGL(kern, dim3(want < cap ? want : cap), dim3(kBlock), 0, s,
+                     static_cast<const T*>(in), static_cast<T*>(out), g, fp.n_orbits,
+                     static_cast<const T*>(pf), fp.offs);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+template <typename T, int VL, bool NT>
+static int dispatch(const FusedPlan& fp, const Geom& g, const void* in, void* out,
+                    const void* pf, hipStream_t s) {
+  const int key = (fp.vi == VL ? 2 : 0) | (fp.vo == VL ? 1 : 0);
+  if (fp.D == 64) {
+    if (fp.vn > 1) return launch64<T, 1, 1, VL, NT>(fp, g, in, out, pf, s);
+    switch (key) {
+      case 0: return launch64<T, 1, 1, 1, NT>(fp, g, in, out, pf, s);
+      case 1: return launch64<T, 1, VL, 1, NT>(fp, g, in, out, pf, s);
+      case 2: return launch64<T, VL, 1, 1, NT>(fp, g, in, out, pf, s);
+      default: return launch64<T, VL, VL, 1, NT>(fp, g, in, out, pf, s);
+    }
   }
-  set_error("internal: no MFMA kernel for D = %d", D);
-  return QSV_ENOTSUP;
+  switch (key) {
+    case 0: return launch256<T, 1, 1, NT>(fp, g, in, out, pf, s);
+    case 1: return launch256<T, 1, VL, NT>(fp, g, in, out, pf, s);
+    case 2: return launch256<T, VL, 1, NT>(fp, g, in, out, pf, s);
+    default: return launch256<T, VL, VL, NT>(fp, g, in, out, pf, s);
+  }
+}
+
+int launch_fused(int dtype, const FusedPlan& fp, const Geom& g, bool nt, const void* in,
+                 void* out, const void* pf, hipStream_t s) {
+  if (fp.D != 64 && fp.D != 256) {
+    set_error("internal: no MFMA kernel for D = %d", fp.D);
+    return QSV_ENOTSUP;
+  }
+  if (fp.n_orbits == 0) return QSV_OK;
+  if (dtype == QSV_F64)
+    return nt ? dispatch<double, 2, true>(fp, g, in, out, pf, s)
+              : dispatch<double, 2, false>(fp, g, in, out, pf, s);
+  return nt ? dispatch<float, 4, true>(fp, g, in, out, pf, s)
+            : dispatch<float, 4, false>(fp, g, in, out, pf, s);
 }
 
 }  // namespace qsv

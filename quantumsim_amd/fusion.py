@@ -1,0 +1,133 @@
+"""Gate fusion: turn a gate list into fewer, denser PTM applications.
+
+A circuit applied gate by gate reads and writes the whole Pauli vector once
+per gate.  Runs of gates that together act on at most ``max_qudits``
+qudits compose into one dense PTM (the reference builds such PTMs with
+``_Chain.ptm``, quantumsim/operations/operation.py:610-623, and applies a
+3-qubit one in test_operations.py:174-195), applied in a single pass: the
+16x16 streaming kernel for 2 qudits, the MFMA kernel for 3-4 qubits.
+
+The pass keeps several open blocks on disjoint qudit sets (gates on
+disjoint qudits commute, so their blocks can stay open side by side).  A
+gate touching blocks whose union with its own qudits still fits
+``max_qudits`` merges them and joins; otherwise those blocks are emitted
+and the gate opens a new block.  At the end every open block is emitted.
+Square PTMs (same input and output dimension per qudit) only.
+"""
+import numpy as np
+
+__all__ = ['fuse', 'FusedGate', 'embed_ptm']
+
+
+class FusedGate:
+    """A dense PTM over ``qudits`` (ascending), layout (out..., in...)."""
+    __slots__ = ('ptm', 'qudits', 'n_source')
+
+    def __init__(self, ptm, qudits, n_source):
+        self.ptm = ptm
+        self.qudits = tuple(qudits)
+        self.n_source = n_source
+
+    def __iter__(self):          # (ptm, qudits) unpacking, like the inputs
+        yield self.ptm
+        yield self.qudits
+
+
+def embed_ptm(ptm, qudits, block, dims):
+    """PTM of ``ptm`` (acting on ``qudits`` in that order) as a square
+    matrix over ``block`` (sorted), identity on the other block qudits."""
+    k = len(qudits)
+    pos = [block.index(q) for q in qudits]
+    m = len(block)
+    din = [dims[q] for q in block]
+    p = np.asarray(ptm, dtype=np.float64).reshape([dims[q] for q in qudits] * 2)
+    # einsum labels: output axes 0..m-1, input axes m..2m-1
+    args = [p, [pos[j] for j in range(k)] + [m + pos[j] for j in range(k)]]
+    for b in range(m):
+        if b not in pos:
+            args += [np.eye(din[b]), [b, m + b]]
+    out = np.einsum(*args, list(range(2 * m)), optimize=True)
+    D = int(np.prod(din))
+    return out.reshape(D, D)
+
+
+class _Block:
+    __slots__ = ('qudits', 'mat', 'n')
+
+    def __init__(self, qudits, mat, n):
+        self.qudits = qudits      # sorted tuple
+        self.mat = mat            # D x D in the block's qudit order
+        self.n = n
+
+
+def _merge(blocks, dims):
+    """Blocks on disjoint qudits -> one block (their PTMs commute)."""
+    qs = tuple(sorted(q for b in blocks for q in b.qudits))
+    mat = np.eye(int(np.prod([dims[q] for q in qs])))
+    for b in blocks:
+        mat = embed_ptm(b.mat.reshape([dims[q] for q in b.qudits] * 2), b.qudits, qs,
+                        dims) @ mat
+    return _Block(qs, mat, sum(b.n for b in blocks))
+
+
+def fuse(gates, dims, max_qudits=2):
+    """Fuse ``gates`` = [(ptm, qudits), ...] on a vector with per-qudit Pauli
+    dimensions ``dims``.  Returns a list of FusedGate whose sequential
+    application equals that of ``gates``."""
+    open_blocks = {}              # qudit -> block
+    out = []
+
+    def emit(b):
+        shape = [dims[q] for q in b.qudits] * 2
+        out.append(FusedGate(b.mat.reshape(shape), b.qudits, b.n))
+        for q in b.qudits:
+            del open_blocks[q]
+
+    for ptm, qudits in gates:
+        qudits = tuple(int(q) for q in qudits)
+        p = np.asarray(ptm)
+        k = len(qudits)
+        if p.shape[:k] != p.shape[k:] or any(p.shape[j] != dims[q] for j, q in enumerate(qudits)):
+            raise ValueError('fuse() needs square PTMs matching the qudit dimensions')
+        touched = []
+        for q in qudits:
+            b = open_blocks.get(q)
+            if b is not None and b not in touched:
+                touched.append(b)
+        union = set(qudits)
+        for b in touched:
+            union.update(b.qudits)
+        if len(union) <= max_qudits:
+            if touched:
+                base = _merge(touched, dims) if len(touched) > 1 else touched[0]
+                for b in touched:
+                    for q in b.qudits:
+                        open_blocks.pop(q, None)
+            else:
+                base = None
+            qs = tuple(sorted(union))
+            if base is None:
+                mat = embed_ptm(p, qudits, qs, dims)
+                n = 1
+            else:
+                prev = base.mat
+                if base.qudits != qs:
+                    prev = embed_ptm(prev.reshape([dims[q] for q in base.qudits] * 2),
+                                     base.qudits, qs, dims)
+                mat = embed_ptm(p, qudits, qs, dims) @ prev
+                n = base.n + 1
+            nb = _Block(qs, mat, n)
+        else:
+            for b in touched:
+                emit(b)
+            qs = tuple(sorted(qudits))
+            nb = _Block(qs, embed_ptm(p, qudits, qs, dims), 1)
+        for q in nb.qudits:
+            open_blocks[q] = nb
+    seen = []
+    for b in open_blocks.values():
+        if b not in seen:
+            seen.append(b)
+    for b in seen:
+        emit(b)
+    return out

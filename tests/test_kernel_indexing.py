@@ -77,7 +77,12 @@ GEOMS += [([2] * 6, (5, 2, 0)), ([2] * 6, (0, 1, 2, 3)), ([2] * 7, (6, 5, 4, 3))
           ([3, 4, 2, 4, 4], (4, 3)), ([1, 4, 4, 1], (3, 2)), ([4, 4, 4, 4, 4, 4], (4,)),
           # dense 64 x 64 and 256 x 256 PTMs (MFMA path)
           ([4] * 5, (0, 1, 2)), ([4] * 5, (4, 2, 0)), ([4] * 6, (1, 2, 3, 4)),
-          ([4] * 4, (3, 0, 1, 2)), ([2] * 8, (7, 0, 3, 1, 5, 2)), ([4, 4, 4, 4, 3], (0, 1, 2))]
+          ([4] * 4, (3, 0, 1, 2)), ([2] * 8, (7, 0, 3, 1, 5, 2)), ([4, 4, 4, 4, 3], (0, 1, 2)),
+          # innermost axis = lowest PTM index: vector accesses; short tails
+          ([4] * 5, (2, 3, 4)), ([4] * 6, (2, 3, 4, 5)), ([4] * 3, (0, 1, 2)),
+          ([4] * 4, (0, 1, 2, 3)), ([4] * 4, (1, 2, 3)), ([16, 4, 4], (1, 2)),
+          ([4, 4, 16], (1, 2)), ([4, 16, 4], (0, 1, 2)), ([9, 4, 4, 4], (1, 2, 3)),
+          ([4, 4, 4, 9], (0, 1, 2)), ([4] * 5, (4, 0, 1))]
 
 
 @pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
@@ -109,14 +114,21 @@ def test_paths_taken():
     assert emulate(nat.QSV_F64, pv, ptm, (2, 3))[1] == 1
     big = rng.standard_normal([9] * 4)
     assert emulate(nat.QSV_F64, big, rng.standard_normal([9] * 4), (1, 2))[1] == 3
-    assert emulate(nat.QSV_F64, pv, rng.standard_normal([4] * 6), (1, 2, 3))[1] == 4
-    assert emulate(nat.QSV_F32, pv, rng.standard_normal([4] * 8), (0, 7, 3, 4))[1] == 4
+    # MFMA path; bits 4-7 / 8-11: elements per load / store access along the
+    # PTM index, bits 12-15: orbits per lane vector
+    assert emulate(nat.QSV_F64, pv, rng.standard_normal([4] * 6), (1, 2, 3))[1] == 4 | 16 | 256 | 2 << 12
+    assert emulate(nat.QSV_F32, pv, rng.standard_normal([4] * 6), (1, 2, 3))[1] == 4 | 16 | 256 | 4 << 12
+    assert emulate(nat.QSV_F32, pv, rng.standard_normal([4] * 8), (0, 7, 3, 4))[1] == 4 | 16 | 256 | 1 << 12
+    assert emulate(nat.QSV_F64, pv, rng.standard_normal([4] * 6), (5, 6, 7))[1] == 4 | 32 | 512 | 1 << 12
+    assert emulate(nat.QSV_F32, pv, rng.standard_normal([4] * 8), (4, 5, 6, 7))[1] == 4 | 64 | 1024 | 1 << 12
+    assert emulate(nat.QSV_F32, pv, rng.standard_normal([4] * 6), (7, 6, 5))[1] == 4 | 16 | 256 | 1 << 12
 
 
 @pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
 @pytest.mark.parametrize('dims,targets', [([4] * 6, (0, 1)), ([4] * 6, (5, 4)), ([4] * 6, (2,)),
                                           ([4] * 6, (0, 5)), ([4] * 5, (1, 2, 3)),
-                                          ([4] * 5, (4, 0, 2, 1)), ([3, 4, 2, 4], (3, 1))])
+                                          ([4] * 5, (4, 0, 2, 1)), ([3, 4, 2, 4], (3, 1)),
+                                          ([4] * 5, (2, 3, 4)), ([4] * 6, (1, 3, 4, 5))])
 def test_emulated_kernels_padded_layout(dims, targets, dtype, rng):
     """Padded strides on the outer axes (as the backend lays out large
     vectors): same results, no access to padding (NaN-filled), every logical

@@ -250,19 +250,41 @@ def test_apply_parity_all_positions(name, k, rng):
             assert err < TOL64, (name, targets, dout, err)
 
 
+@pytest.mark.parametrize('n', [4, 6, 9])
 @pytest.mark.parametrize('k', [3, 4])
-def test_apply_fused_dense(k, rng):
+def test_apply_fused_dense(n, k, rng):
     # 64 x 64 and 256 x 256 PTMs (test_operations.py:174-195 applies 64 x 64)
-    n = 6
+    # on the MFMA kernels: outer, inner (vector accesses) and scattered
+    # targets; n = 9 gives several tiles per wave
     bs = [bases.general(2)] * n
     pv = rng.standard_normal([4] * n)
     for targets in [tuple(range(k)), tuple(range(n - k, n)), tuple(rng.permutation(n)[:k])]:
-        ptm = rng.standard_normal([4] * (2 * k))
-        v = qa.PauliVectorHIP(bs, pv.copy())
+        ptm = rng.standard_normal([4] * (2 * k)) / 4 ** (k / 2)
         o = PauliVectorNumpy(_oracle_bases(bs), pv.copy())
-        v.apply_ptm(ptm, *targets)
         o.apply_ptm(ptm, *targets)
-        assert rel_err(v.to_pv(), o.to_pv()) < TOL64
+        v = qa.PauliVectorHIP(bs, pv.copy())
+        v.apply_ptm(ptm, *targets)
+        assert rel_err(v.to_pv(), o.to_pv()) < TOL64, (n, targets)
+        v32 = qa.PauliVectorHIP(bs, pv.copy(), dtype='float32')
+        v32.apply_ptm(ptm, *targets)
+        assert rel_err(v32.to_pv(), o.to_pv()) < 1e-5, (n, targets)
+
+
+def test_apply_fused_dense_ququarts(rng):
+    # a Pauli dimension of 16 (d = 4 qudit) inside 64- and 256-wide PTMs
+    bs = [bases.general(2), bases.general(4), bases.general(2), bases.general(2),
+          bases.general(4)]
+    dims = [b.dim_pauli for b in bs]
+    pv = rng.standard_normal(dims)
+    for targets in [(1, 2), (3, 4), (0, 1, 2), (2, 3, 4), (4, 0)]:
+        din = [dims[t] for t in targets]
+        ptm = rng.standard_normal(din + din) / np.sqrt(np.prod(din))
+        o = PauliVectorNumpy(_oracle_bases(bs), pv.copy())
+        o.apply_ptm(ptm, *targets)
+        for dt, tol in (('float64', TOL64), ('float32', 1e-5)):
+            v = qa.PauliVectorHIP(bs, pv.copy(), dtype=dt)
+            v.apply_ptm(ptm, *targets)
+            assert rel_err(v.to_pv(), o.to_pv()) < tol, (targets, dt)
 
 
 def test_apply_qutrit_pair_81(rng):
