@@ -16,7 +16,9 @@ Printed JSON line (stdout):
                reference backend) applying the same 2-qubit PTMs to a
                14-qubit state on this host's cores, bounded sample.
 Extra fields: float32 rate, per-position kernel GB/s, 1-qubit rate, the
-rate of whole circuit layers, and the diagonal readout time.
+rate of whole circuit layers, the diagonal readout time, the whole 60-layer
+circuit gate by gate and fused into dense 2-/3-qubit PTMs, the batched
+forward + adjoint of 512 circuits and the differentiable 12-qubit step.
 """
 import argparse
 import json
@@ -156,6 +158,45 @@ def bench_batched(args):
     log('batched: %s' % res)
     del cb
     torch.cuda.empty_cache()
+    return res
+
+
+def bench_circuit(args, circuit, n):
+    """The whole seeded 60-layer circuit on the 14-qubit state: gate by gate
+    against gates fused into dense PTMs of up to 2 and 3 qubits (the 3-qubit
+    blocks run on the MFMA kernel).  Fusion is host work done once per
+    circuit, outside the timed region (timed separately)."""
+    import torch
+    import quantumsim_amd as qa
+    from quantumsim_amd import bases
+    from quantumsim_amd.fusion import fuse
+    stream = torch.cuda.current_stream()
+    plans = {1: [(p, tuple(q)) for p, q in circuit]}
+    fuse_s = {}
+    for k in (2, 3):
+        t0 = time.perf_counter()
+        plans[k] = [(f.ptm, f.qudits) for f in fuse(circuit, [4] * n, k)]
+        fuse_s[k] = time.perf_counter() - t0
+    res = {'gates': len(circuit), 'layers': 60}
+    for dtype in ('float64', 'float32'):
+        v = qa.PauliVectorHIP([bases.general(2)] * n, force=True, dtype=dtype)
+        row = {}
+        for k, gates in plans.items():
+            def run(gates=gates):
+                for p, q in gates:
+                    v.apply_ptm(p, *q)
+            run()
+            sec = timed(run, 1, stream)
+            row['fuse%d' % k if k > 1 else 'gate_by_gate'] = {
+                'applications': len(gates), 'ms': round(sec * 1e3, 2),
+                'gates_per_s': round(len(circuit) / sec, 1)}
+        res[dtype[:1] + dtype[-2:]] = row
+        del v
+        torch.cuda.empty_cache()
+    res['fuse_host_s'] = {'fuse2': round(fuse_s[2], 3), 'fuse3': round(fuse_s[3], 3)}
+    res['speedup_fuse3_vs_gate_by_gate'] = {
+        d: round(res[d]['gate_by_gate']['ms'] / res[d]['fuse3']['ms'], 2) for d in ('f64', 'f32')}
+    log('circuit: %s' % res)
     return res
 
 
@@ -300,6 +341,7 @@ def main():
                         'f32': round(res['float32']['diag_us'], 1)},
     }
     if not args.no_extras:
+        out['circuit'] = bench_circuit(args, circuit, n)
         out['batched'] = bench_batched(args)
         out['differentiable'] = bench_differentiable(args)
     if not args.no_cpu:

@@ -9,14 +9,16 @@ qudits compose into one dense PTM (the reference builds such PTMs with
 
 The pass keeps several open blocks on disjoint qudit sets (gates on
 disjoint qudits commute, so their blocks can stay open side by side).  A
-gate touching blocks whose union with its own qudits still fits
-``max_qudits`` merges them and joins; otherwise those blocks are emitted
-and the gate opens a new block.  At the end every open block is emitted.
+gate touching open blocks joins the largest subset of them whose union with
+its own qudits fits ``max_qudits`` (ties: the subset holding more gates);
+the touched blocks left out are emitted first (they act on a qudit of the
+gate, so they must precede it).  At the end every open block is emitted.
 Square PTMs (same input and output dimension per qudit) only.
 """
+import itertools
 import numpy as np
 
-__all__ = ['fuse', 'FusedGate', 'embed_ptm']
+__all__ = ['fuse', 'FusedGate', 'embed_ptm', 'apply_circuit']
 
 
 class FusedGate:
@@ -94,33 +96,32 @@ def fuse(gates, dims, max_qudits=2):
             b = open_blocks.get(q)
             if b is not None and b not in touched:
                 touched.append(b)
-        union = set(qudits)
+        # largest subset of the touched blocks that fits together with the gate
+        best, best_key = (), None
+        for r in range(len(touched), -1, -1):
+            for sub in itertools.combinations(touched, r):
+                u = set(qudits).union(*(b.qudits for b in sub))
+                if len(u) > max_qudits and sub:
+                    continue
+                key = (len(u) if sub else 0, sum(b.n for b in sub))
+                if best_key is None or key > best_key:
+                    best, best_key = sub, key
         for b in touched:
-            union.update(b.qudits)
-        if len(union) <= max_qudits:
-            if touched:
-                base = _merge(touched, dims) if len(touched) > 1 else touched[0]
-                for b in touched:
-                    for q in b.qudits:
-                        open_blocks.pop(q, None)
-            else:
-                base = None
-            qs = tuple(sorted(union))
-            if base is None:
-                mat = embed_ptm(p, qudits, qs, dims)
-                n = 1
-            else:
-                prev = base.mat
-                if base.qudits != qs:
-                    prev = embed_ptm(prev.reshape([dims[q] for q in base.qudits] * 2),
-                                     base.qudits, qs, dims)
-                mat = embed_ptm(p, qudits, qs, dims) @ prev
-                n = base.n + 1
-            nb = _Block(qs, mat, n)
-        else:
-            for b in touched:
+            if b not in best:
                 emit(b)
-            qs = tuple(sorted(qudits))
+        union = set(qudits).union(*(b.qudits for b in best))
+        qs = tuple(sorted(union))
+        if best:
+            base = _merge(list(best), dims) if len(best) > 1 else best[0]
+            for b in best:
+                for q in b.qudits:
+                    open_blocks.pop(q, None)
+            prev = base.mat
+            if base.qudits != qs:
+                prev = embed_ptm(prev.reshape([dims[q] for q in base.qudits] * 2),
+                                 base.qudits, qs, dims)
+            nb = _Block(qs, embed_ptm(p, qudits, qs, dims) @ prev, base.n + 1)
+        else:
             nb = _Block(qs, embed_ptm(p, qudits, qs, dims), 1)
         for q in nb.qudits:
             open_blocks[q] = nb
@@ -131,3 +132,17 @@ def fuse(gates, dims, max_qudits=2):
     for b in seen:
         emit(b)
     return out
+
+
+def apply_circuit(pv, gates, max_qudits=3):
+    """Apply ``gates`` = [(ptm, qudits), ...] (PTMs in ``pv``'s bases) to the
+    Pauli vector ``pv``, fused into blocks of up to ``max_qudits`` qudits
+    (0 or 1: gate by gate).  Returns the number of PTM applications made."""
+    if max_qudits is None or max_qudits <= 1:
+        for ptm, qudits in gates:
+            pv.apply_ptm(np.asarray(ptm), *qudits)
+        return len(gates)
+    fused = fuse(gates, list(pv.dim_pauli), max_qudits)
+    for f in fused:
+        pv.apply_ptm(f.ptm, *f.qudits)
+    return len(fused)

@@ -5,6 +5,8 @@ run gate by gate through PauliVectorHIP and through the float64 oracle.
   trace within 1e-12 in float64; diagonal within 1e-5 absolute in float32.
 * the 14-qubit, 60-layer brickwork circuit restricted to qubits 0-9 (the
   10-qubit slice), checked the same way.
+* the same circuits fused into dense 2-, 3- and 4-qubit PTMs (MFMA kernels
+  for 3 and 4) against the gate-by-gate oracle.
 """
 import numpy as np
 import pytest
@@ -12,6 +14,7 @@ import pytest
 import quantumsim_amd as qa
 from quantumsim_amd import bases
 from quantumsim_amd.workloads import noisy_layers
+from quantumsim_amd.fusion import apply_circuit
 from oracle.pauli_vector import PauliVectorNumpy
 from oracle import bases as ob
 
@@ -51,3 +54,25 @@ def test_circuit_parity(n, layers, pairing, seed):
     assert err32 < 1e-5
     assert abs(v32.trace() - t_ref) < 1e-5
     assert abs(v32.renormalize() - t_ref) < 1e-5
+
+
+@pytest.mark.parametrize('max_qudits', [2, 3, 4])
+@pytest.mark.parametrize('n,layers,pairing,seed', [(6, 40, 'random', 0),
+                                                   (10, 60, 'brickwork', 1)])
+def test_fused_circuit_parity(n, layers, pairing, seed, max_qudits):
+    circuit, _ = noisy_layers(n, layers, seed, pairing)
+    o = _run_oracle(circuit, n)
+    d_ref = o.diagonal()
+    v = qa.PauliVectorHIP([bases.general(2)] * n, force=True)
+    napp = apply_circuit(v, circuit, max_qudits)
+    assert napp < len(circuit)
+    err64 = np.max(np.abs(v.to_pv() - o.to_pv()))
+    print('n=%d fused<=%d: %d applications for %d gates, fp64 max |pv err| = %.3e'
+          % (n, max_qudits, napp, len(circuit), err64))
+    assert err64 < 1e-12
+    assert abs(v.trace() - o.trace()) < 1e-12
+    v32 = qa.PauliVectorHIP([bases.general(2)] * n, force=True, dtype='float32')
+    apply_circuit(v32, circuit, max_qudits)
+    err32 = np.max(np.abs(v32.diagonal() - d_ref))
+    print('n=%d fused<=%d fp32 max |diag err| = %.3e' % (n, max_qudits, err32))
+    assert err32 < 1e-5
