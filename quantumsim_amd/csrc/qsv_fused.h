@@ -85,8 +85,11 @@ inline void fused_fragments(int D, int elem_bytes, int vi, int vo, const double*
       }
 }
 
-// k-steps per LDS chunk of the 256 x 256 kernel (4096 fragment entries)
-constexpr int kFusedChunkSteps = 4;
+// k-steps per LDS chunk of the 256 x 256 kernel: a chunk covers 16 (float64)
+// or 32 (float32) consecutive PTM columns, i.e. one whole 128-byte line of an
+// orbit's row when the targets are the innermost axes, so each line is read
+// once (with 16 float32 columns half of every line was fetched twice).
+constexpr int fused_chunk_steps(int elem_bytes) { return elem_bytes == 8 ? 4 : 8; }
 
 int launch_fused(int dtype, const FusedPlan& fp, const Geom& g, bool nt, const void* in,
                  void* out, const void* ptm_frag, hipStream_t s);

@@ -3,16 +3,20 @@
 // described in qsv_fused.h.
 //
 // k_fused64: the 64 x 64 PTM (32 KB in float64) is staged in LDS once per
-// block; after that every wave runs on its own: it walks tiles of 16 NB
-// orbits, loading the next tile's B fragments while the MFMAs of the
-// current one run (two register buffers), then stores the accumulators.
+// block; after that every wave runs on its own, walking tiles of 16 NB
+// orbits: load the tile's B fragments straight from HBM (16-byte vectors
+// along orbits or along the PTM index), 16 NB x 4 MFMAs per k-step with the
+// A fragments read from LDS, store the accumulators straight back.  Few
+// registers (60-90 VGPRs) leave room for 3-5 waves per SIMD, which overlap
+// one wave's memory wait with another's MFMAs (measured: as fast as
+// explicit double-buffering in registers, which halves the occupancy).
 // Memory-bound: 2 D element accesses per orbit against 2 D^2 flops.
 //
 // k_fused256: the 256 x 256 PTM (512 KB in float64) does not fit in LDS; a
 // block of four waves takes 64 orbits (16 per wave, accumulators for all
-// 256 rows in registers) and streams the PTM through LDS in chunks of four
-// k-steps (4096 entries), double-buffered: the next chunk and the next B
-// fragments are loaded while the current chunk's 64 MFMAs per wave run.
+// 256 rows in registers) and streams the PTM through LDS in chunks of 4
+// (float64) or 8 (float32) k-steps, double-buffered: the next chunk and the
+// next B fragments are loaded while the current chunk's MFMAs run.
 // MFMA-bound.
 #include "qsv_fused.h"
 
@@ -74,6 +78,15 @@ __device__ __forceinline__ void store_vec(T* p, const T* vals) {
 //   VN > 1: tile * 16 VN + VN n + nb   (NB = VN: lane n owns VN consecutive
 //           orbits, loaded and stored as one 16-byte vector when the
 //           innermost axis is a bystander with unit stride)
+
+// NT (nontemporal) applies to the loads; stores are nontemporal only when
+// every store instruction writes whole 128-byte lines (16 consecutive
+// orbits of float64, or orbit vectors): partial lines written nontemporally
+// reach HBM as separate masked writes (measured: up to 2.2x the bytes).
+template <typename T, int VI, int VO, int VN, bool NT>
+__host__ __device__ constexpr bool fused_nt_stores() {
+  return NT && (VN > 1 || (VI == 1 && VO == 1 && sizeof(T) == 8));
+}
 
 template <typename T, int VI, int VO, int VN, bool NT>
 __global__ void __launch_bounds__(kBlock)
@@ -147,7 +160,7 @@ __global__ void __launch_bounds__(kBlock)
         VNv v;
 #pragma unroll
         for (int e = 0; e < VN; ++e) v[e] = acc[q >> 2][e][q & 3];
-        gstore<NT>(reinterpret_cast<VNv*>(p + fo.out_u[q]), v);
+        gstore<fused_nt_stores<T, VI, VO, VN, NT>()>(reinterpret_cast<VNv*>(p + fo.out_u[q]), v);
       }
     } else {
 #pragma unroll
@@ -158,7 +171,7 @@ __global__ void __launch_bounds__(kBlock)
           const int q = v * VO + e;
           vals[e] = acc[q >> 2][l][q & 3];
         }
-        store_vec<T, VO, NT>(p + fo.out_u[v], vals);
+        store_vec<T, VO, fused_nt_stores<T, VI, VO, VN, NT>()>(p + fo.out_u[v], vals);
       }
     }
   };
@@ -184,7 +197,7 @@ template <typename T, int VI, int VO, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_fused256(const T* __restrict__ in, T* __restrict__ out, const Geom g,
                const uint32_t n_orbits, const T* __restrict__ pf, const FusedOffs fo) {
-  constexpr int MB = 16, KS = 64, CS = kFusedChunkSteps, NCH = KS / CS;
+  constexpr int MB = 16, KS = 64, CS = fused_chunk_steps(sizeof(T)), NCH = KS / CS;
   constexpr int CHUNK = CS * MB * 64;
   constexpr int PER = CHUNK / kBlock;
   constexpr uint32_t COLS = 16 * (kBlock / 64);
@@ -249,7 +262,7 @@ __global__ void __launch_bounds__(kBlock)
           const int q = v * VO + e;
           vals[e] = acc[q >> 2][q & 3];
         }
-        store_vec<T, VO, NT>(p + fo.out_u[v], vals);
+        store_vec<T, VO, fused_nt_stores<T, VI, VO, 1, NT>()>(p + fo.out_u[v], vals);
       }
     }
   }
