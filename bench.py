@@ -116,11 +116,14 @@ def bench_batched(args):
     import torch
     from quantumsim_amd.batch import CircuitBatch, plain_forward_backward
     from quantumsim_amd.workloads import random_circuit_batch
+    from quantumsim_amd.autotune import Autotuner, set_tuner
     circuits = random_circuit_batch(args.circuits, seed=2024)
+    tuner = Autotuner(path=None)                 # tuned in this run, not read from disk
+    set_tuner(tuner)
     t0 = time.perf_counter()
     cb = CircuitBatch(circuits, dtype=torch.float32, checkpoint_every=16)
     w = cb.random_weights(7)
-    cb.capture(w)
+    cb.capture(w, autotune=True)
     t_setup = time.perf_counter() - t0
     stream = cb.stream
     reps = 3
@@ -154,7 +157,8 @@ def bench_batched(args):
            'plain_circuits_per_s': round(len(circuits) / plain_sec, 2),
            'speedup_vs_plain': round(plain_sec / sec, 2),
            'plain_sample': len(sample), 'dtype': 'f32 state, f64 readout',
-           'checkpoint_every': 16}
+           'checkpoint_every': 16, 'setup_s': round(t_setup, 1),
+           'autotune': tuner.summary()}
     log('batched: %s' % res)
     del cb
     torch.cuda.empty_cache()

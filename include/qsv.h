@@ -35,6 +35,21 @@ extern "C" {
 #define QSV_MAX_RANK 32    /* qudits per Pauli vector                    */
 #define QSV_MAX_TARGETS 8  /* qudits per PTM                             */
 
+/* Launch shape of the batched apply and of the PTM gradient, as chosen by
+ * an autotuner (quantumsim_amd/autotune.py times the candidates on the
+ * shapes at hand).  Passing NULL, or zero fields, keeps the library's own
+ * planning.  Plain data: the library keeps no tuning state. */
+#define QSV_KERNEL_AUTO 0    /* planner's choice                          */
+#define QSV_KERNEL_DIRECT 1  /* orbit-streaming kernel                    */
+#define QSV_KERNEL_LDS 2     /* LDS-tiled kernel (QSV_ENOTSUP if the      */
+                             /* geometry has no tiling)                   */
+typedef struct qsv_tile {
+  int32_t kernel;       /* QSV_KERNEL_*                                    */
+  int32_t tile_orbits;  /* LDS tile size in orbits (0: planner's)        */
+  int32_t grad_blocks;  /* gradient blocks per state, <= 256 (0: auto)   */
+  int32_t reserved;     /* must be 0                                     */
+} qsv_tile;
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* qsv_last_error(void);
 
@@ -98,7 +113,14 @@ int qsv_apply_ptm_batched(int dtype, const void* in, void* out, int batch, int r
                           const int32_t* dims_out_t, const void* ptm_dev,
                           int64_t ptm_batch_stride, void* stream);
 
-/* Bytes of device workspace qsv_ptm_grad needs. */
+/* qsv_apply_ptm_batched with an explicit launch shape (NULL = planner's). */
+int qsv_apply_ptm_batched_tiled(int dtype, const void* in, void* out, int batch, int rank,
+                                const int64_t* dims, int k, const int32_t* targets,
+                                const int32_t* dims_out_t, const void* ptm_dev,
+                                int64_t ptm_batch_stride, const qsv_tile* tile, void* stream);
+
+/* Bytes of device workspace qsv_ptm_grad / qsv_ptm_grad_tiled need (enough
+ * for any launch shape). */
 int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64_t* dims, int k,
                                      const int32_t* targets, const int32_t* dims_out_t);
 
@@ -116,6 +138,28 @@ int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batch, int rank
                  const int64_t* dims, int k, const int32_t* targets, const int32_t* dims_out_t,
                  void* dptm, int accumulate, void* workspace, int64_t workspace_bytes,
                  void* stream);
+
+/* qsv_ptm_grad with an explicit launch shape (tile->grad_blocks; NULL =
+ * automatic).  The result does not depend on the shape beyond the order of
+ * the float64 partial sums. */
+int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, int batch, int rank,
+                       const int64_t* dims, int k, const int32_t* targets,
+                       const int32_t* dims_out_t, void* dptm, int accumulate, void* workspace,
+                       int64_t workspace_bytes, const qsv_tile* tile, void* stream);
+
+/* CPU replay of qsv_ptm_grad for one state of at most 2^24 entries: the
+ * same plan, every lane's loads checked against the buffers (and vector
+ * alignment), the MFMA contraction in hardware index order, the block
+ * reduction and the fixed-order final sum, each orbit checked to be used
+ * exactly once.  x, ybar: host float64 arrays (contiguous); dptm: host
+ * float64 [dout * din].  grad_blocks: blocks per state, as the qsv_tile field (0:
+ * automatic for a batch of one).  *vn_out (may be NULL) = orbits per lane
+ * vector the GPU call would use for `dtype`.  No counterpart in the reference; it lets
+ * CPU tests pin the gradient kernel's indexing before GPU use. */
+int qsv_emulate_ptm_grad(int dtype, const double* x, const double* ybar, int rank,
+                         const int64_t* dims, int k, const int32_t* targets,
+                         const int32_t* dims_out_t, double* dptm, int32_t grad_blocks,
+                         int32_t* vn_out);
 
 /* out = in[np.ix_(idx_0, ..., idx_{rank-1})] with an index of -1 producing
  * 0.  The index lists live in device memory as int32: idx_dev[offsets[r] +
@@ -180,6 +224,14 @@ int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, int rank,
                           const int64_t* out_strides, int k, const int32_t* targets,
                           const int32_t* dims_out_t, const double* ptm_f64,
                           int32_t* path_out);
+
+/* qsv_emulate_apply_ptm of the batched path's kernel choice for one state,
+ * with an explicit launch shape (the autotuner's candidates are replayed
+ * with it).  QSV_ENOTSUP when the tile is infeasible for the geometry. */
+int qsv_emulate_apply_ptm_tiled(int dtype, const double* in, double* out, int rank,
+                                const int64_t* dims, int k, const int32_t* targets,
+                                const int32_t* dims_out_t, const double* ptm_f64,
+                                const qsv_tile* tile, int32_t* path_out);
 
 #ifdef __cplusplus
 }

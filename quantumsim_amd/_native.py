@@ -11,7 +11,7 @@ import os
 
 import torch  # noqa: F401  (must precede the CDLL load, see module doc)
 
-__all__ = ['lib', 'check', 'QSV_F32', 'QSV_F64', 'NativeError', 'library_path',
+__all__ = ['lib', 'check', 'QSV_F32', 'QSV_F64', 'NativeError', 'library_path', 'Tile',
            'dtype_code']
 
 QSV_F32 = 0
@@ -28,6 +28,19 @@ class NativeError(RuntimeError):
 _c_i32p = ctypes.POINTER(ctypes.c_int32)
 _c_i64p = ctypes.POINTER(ctypes.c_int64)
 _c_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class Tile(ctypes.Structure):
+    """qsv_tile (include/qsv.h): launch shape of the batched apply and the
+    PTM gradient chosen by the autotuner; zero fields = library's choice."""
+    _fields_ = [('kernel', ctypes.c_int32), ('tile_orbits', ctypes.c_int32),
+                ('grad_blocks', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+
+    def key(self):
+        return (self.kernel, self.tile_orbits, self.grad_blocks)
+
+
+KERNEL_AUTO, KERNEL_DIRECT, KERNEL_LDS = 0, 1, 2
 _vp = ctypes.c_void_p
 
 _SIGNATURES = {
@@ -52,6 +65,18 @@ _SIGNATURES = {
     'qsv_ptm_grad': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_i64p,
                                     ctypes.c_int, _c_i32p, _c_i32p, _vp, ctypes.c_int, _vp,
                                     ctypes.c_int64, _vp]),
+    'qsv_emulate_ptm_grad': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, _c_i64p,
+                                            ctypes.c_int, _c_i32p, _c_i32p, _c_f64p,
+                                            ctypes.c_int32, _c_i32p]),
+    'qsv_ptm_grad_tiled': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                          _c_i64p, ctypes.c_int, _c_i32p, _c_i32p, _vp,
+                                          ctypes.c_int, _vp, ctypes.c_int64, _vp, _vp]),
+    'qsv_apply_ptm_batched_tiled': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                                   ctypes.c_int, _c_i64p, ctypes.c_int, _c_i32p,
+                                                   _c_i32p, _vp, ctypes.c_int64, _vp, _vp]),
+    'qsv_emulate_apply_ptm_tiled': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int,
+                                                   _c_i64p, ctypes.c_int, _c_i32p, _c_i32p,
+                                                   _c_f64p, _vp, _c_i32p]),
     'qsv_multitake': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                      _c_i64p, _c_i64p, _vp, _c_i32p, _vp]),
     'qsv_multitake_strided': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,

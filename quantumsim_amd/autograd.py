@@ -27,6 +27,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from .autotune import get_tuner
 from .bases import general
 
 __all__ = ['apply', 'run_circuit', 'diagonal', 'Gate', 'qubit_basis_tensor', 'kraus_ptm',
@@ -58,9 +59,11 @@ def _apply_raw(x, ptm, shape, targets, dims_out):
     y = torch.empty((B, n_out), dtype=x.dtype, device=x.device)
     ptm = ptm.to(dtype=x.dtype).contiguous()
     pstride = 0 if ptm.shape[0] == 1 else ptm.shape[1] * ptm.shape[2]
-    rc = nat.lib().qsv_apply_ptm_batched(
+    tile = get_tuner().apply_tile(x.dtype, B, shape, targets, dims_out)
+    rc = nat.lib().qsv_apply_ptm_batched_tiled(
         nat.dtype_code(x.dtype), x.data_ptr(), y.data_ptr(), B, len(shape), nat.i64(shape),
-        len(targets), nat.i32(targets), nat.i32(dims_out), ptm.data_ptr(), pstride, _stream())
+        len(targets), nat.i32(targets), nat.i32(dims_out), ptm.data_ptr(), pstride,
+        None if tile is None else nat.ctypes.byref(tile), _stream())
     nat.check(rc, 'apply')
     return y
 
@@ -76,9 +79,11 @@ def _ptm_grad_raw(x, ybar, shape, targets, dims_out):
                    'ptm_grad')
     buf = _Workspace.get(x.device, ws)
     out = torch.empty((B, dout, din), dtype=x.dtype, device=x.device)
-    rc = L.qsv_ptm_grad(nat.dtype_code(x.dtype), x.data_ptr(), ybar.data_ptr(), B, len(shape),
-                        nat.i64(shape), k, nat.i32(targets), nat.i32(dims_out),
-                        out.data_ptr(), 0, buf.data_ptr(), ws, _stream())
+    tile = get_tuner().grad_tile(x.dtype, B, shape, targets, dims_out)
+    rc = L.qsv_ptm_grad_tiled(nat.dtype_code(x.dtype), x.data_ptr(), ybar.data_ptr(), B,
+                              len(shape), nat.i64(shape), k, nat.i32(targets), nat.i32(dims_out),
+                              out.data_ptr(), 0, buf.data_ptr(), ws,
+                              None if tile is None else nat.ctypes.byref(tile), _stream())
     nat.check(rc, 'ptm_grad')
     return out
 

@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from .autograd import run_circuit, diagonal
+from .autotune import tuning
 from .bases import general
 from .workloads import rotation_ptms, damping_ptms, fused_layer_gates
 
@@ -132,9 +133,15 @@ class CircuitBatch:
         return [torch.tensor(g.standard_normal((len(gr.members), 2 ** gr.n)),
                              dtype=torch.float64, device=self.device) for gr in self.groups]
 
-    def capture(self, weights):
+    def capture(self, weights, autotune=True):
         """Record one forward + adjoint step in a HIP graph (after an eager
-        warm-up step that sizes every buffer); replay() re-runs it."""
+        warm-up step that sizes every buffer and, with ``autotune``, times
+        the launch shapes of every batched apply and PTM gradient of the
+        step: see quantumsim_amd.autotune); replay() re-runs it."""
+        if autotune:
+            with tuning():
+                self.step(weights)               # tunes each new geometry once
+            torch.cuda.synchronize(self.device)
         self.step(weights)                       # warm-up: allocations, tables
         torch.cuda.synchronize(self.device)
         self.graph = torch.cuda.CUDAGraph()

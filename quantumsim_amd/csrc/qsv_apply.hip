@@ -521,14 +521,24 @@ static int64_t lds_tile_orbits(int64_t dmax, int elem_bytes) {
   return x < 16 ? int64_t(16) : (x > kLdsMaxOrbits ? int64_t(kLdsMaxOrbits) : x);
 }
 
-static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp) {
+// `tile` (may be NULL) forces the choice: QSV_KERNEL_DIRECT never tiles,
+// QSV_KERNEL_LDS tiles whatever the run length, tile_orbits caps the tile.
+static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp,
+                          const qsv_tile* tile = nullptr) {
   if (p.din > 16 || p.dout > 16) return false;
-  const int64_t cap = lds_tile_orbits(std::max(p.din, p.dout), elem_bytes);
+  if (tile && tile->kernel == QSV_KERNEL_DIRECT) return false;
+  int64_t cap = lds_tile_orbits(std::max(p.din, p.dout), elem_bytes);
+  if (tile && tile->tile_orbits > 0) {
+    cap = tile->tile_orbits;
+    if (std::max(p.din, p.dout) > 4 && cap > 2 * kBlock) cap = 2 * kBlock;
+    cap = std::min<int64_t>(std::max<int64_t>(cap, 16), kLdsMaxOrbits);
+  }
   int t_in = -1;
   for (int j = 0; j < p.k; ++j) t_in = std::max(t_in, int(p.targets[j]));
   int64_t Z = 1;
   for (int q = t_in + 1; q < p.rank; ++q) Z *= p.dims_in[q];
-  if (Z * elem_bytes >= lds_run_threshold()) return false;
+  const bool forced = tile && tile->kernel == QSV_KERNEL_LDS;
+  if (!forced && Z * elem_bytes >= lds_run_threshold()) return false;
   // axis c nests without a gap on the axes below it (input and output)
   auto contig = [&](int c) {
     if (c == p.rank - 1) return p.sin[c] == 1 && p.sout[c] == 1;
@@ -818,14 +828,18 @@ static int run_inline(const Plan& p, const double* ptm, const void* in, void* ou
 
 template <typename T, int CH>
 static int run_ptr(const Plan& p, const void* ptm_dev, const void* in, void* out,
-                   hipStream_t s, int64_t pstride = 0) {
+                   hipStream_t s, int64_t pstride = 0, const qsv_tile* tile = nullptr) {
   PtrOp<T, CH> op;
   memset(&op, 0, sizeof(op));
   op.ptm = static_cast<const T*>(ptm_dev);
   op.pstride = pstride;
   op.ld = p.din;
   LdsPlan lp;
-  if (make_lds_plan(p, sizeof(T), lp)) return dispatch_lds<T, CH>(lp, op, in, out, s);
+  if (make_lds_plan(p, sizeof(T), lp, tile)) return dispatch_lds<T, CH>(lp, op, in, out, s);
+  if (tile && tile->kernel == QSV_KERNEL_LDS) {
+    set_error("no LDS tiling for this geometry");
+    return QSV_ENOTSUP;
+  }
   fill_offsets<T, CH>(p, op.off_in, op.off_out);
   return dispatch_vec_small<T, CH, PtrOp>(p, op, in, out, s);
 }
@@ -1310,10 +1324,32 @@ extern "C" int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, i
   return emulate_direct(p, ptm_f64, in, out);
 }
 
+static int check_tile(const qsv_tile* t) {
+  if (!t) return QSV_OK;
+  QSV_CHECK_ARG(t->kernel >= QSV_KERNEL_AUTO && t->kernel <= QSV_KERNEL_LDS,
+                "tile: unknown kernel %d", t->kernel);
+  QSV_CHECK_ARG(t->tile_orbits >= 0 && t->tile_orbits <= kLdsMaxOrbits,
+                "tile: tile_orbits %d outside [0, %d]", t->tile_orbits, kLdsMaxOrbits);
+  QSV_CHECK_ARG(t->grad_blocks >= 0 && t->grad_blocks <= 256,
+                "tile: grad_blocks %d outside [0, 256]", t->grad_blocks);
+  QSV_CHECK_ARG(t->reserved == 0, "tile: reserved field must be 0");
+  return QSV_OK;
+}
+
 extern "C" int qsv_apply_ptm_batched(int dtype, const void* in, void* out, int batch, int rank,
                                      const int64_t* dims, int k, const int32_t* targets,
                                      const int32_t* dims_out_t, const void* ptm_dev,
                                      int64_t ptm_batch_stride, void* stream) {
+  return qsv_apply_ptm_batched_tiled(dtype, in, out, batch, rank, dims, k, targets, dims_out_t,
+                                     ptm_dev, ptm_batch_stride, nullptr, stream);
+}
+
+extern "C" int qsv_apply_ptm_batched_tiled(int dtype, const void* in, void* out, int batch,
+                                           int rank, const int64_t* dims, int k,
+                                           const int32_t* targets, const int32_t* dims_out_t,
+                                           const void* ptm_dev, int64_t ptm_batch_stride,
+                                           const qsv_tile* tile, void* stream) {
+  if (const int rt = check_tile(tile)) return rt;
   Plan p;
   const int rc = make_plan(dtype, rank, dims, nullptr, nullptr, k, targets, dims_out_t, p);
   if (rc) return rc;
@@ -1329,8 +1365,37 @@ extern "C" int qsv_apply_ptm_batched(int dtype, const void* in, void* out, int b
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool small = std::max(p.din, p.dout) <= 4;
   if (dtype == QSV_F64)
-    return small ? run_ptr<double, 4>(p, ptm_dev, in, out, s, ptm_batch_stride)
-                 : run_ptr<double, 16>(p, ptm_dev, in, out, s, ptm_batch_stride);
-  return small ? run_ptr<float, 4>(p, ptm_dev, in, out, s, ptm_batch_stride)
-               : run_ptr<float, 16>(p, ptm_dev, in, out, s, ptm_batch_stride);
+    return small ? run_ptr<double, 4>(p, ptm_dev, in, out, s, ptm_batch_stride, tile)
+                 : run_ptr<double, 16>(p, ptm_dev, in, out, s, ptm_batch_stride, tile);
+  return small ? run_ptr<float, 4>(p, ptm_dev, in, out, s, ptm_batch_stride, tile)
+               : run_ptr<float, 16>(p, ptm_dev, in, out, s, ptm_batch_stride, tile);
+}
+
+extern "C" int qsv_emulate_apply_ptm_tiled(int dtype, const double* in, double* out, int rank,
+                                           const int64_t* dims, int k, const int32_t* targets,
+                                           const int32_t* dims_out_t, const double* ptm_f64,
+                                           const qsv_tile* tile, int32_t* path_out) {
+  if (const int rt = check_tile(tile)) return rt;
+  Plan p;
+  const int rc = make_plan(dtype, rank, dims, nullptr, nullptr, k, targets, dims_out_t, p);
+  if (rc) return rc;
+  QSV_CHECK_ARG(in && out && ptm_f64, "null pointer argument");
+  QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24) && p.n_out <= (int64_t(1) << 24),
+                "emulation is for small vectors");
+  if (!fits_inline(p.din, p.dout)) {
+    set_error("batched apply supports PTMs up to %d x %d", kInline, kInline);
+    return QSV_ENOTSUP;
+  }
+  const int es = dtype_size(dtype);
+  LdsPlan lp;
+  if (make_lds_plan(p, es, lp, tile)) {
+    if (path_out) *path_out = 2;
+    return emulate_lds(p, lp, es, ptm_f64, in, out);
+  }
+  if (tile && tile->kernel == QSV_KERNEL_LDS) {
+    set_error("no LDS tiling for this geometry");
+    return QSV_ENOTSUP;
+  }
+  if (path_out) *path_out = 1;
+  return emulate_direct(p, ptm_f64, in, out);
 }

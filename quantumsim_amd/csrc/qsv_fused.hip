@@ -19,25 +19,9 @@
 // next B fragments are loaded while the current chunk's MFMAs run.
 // MFMA-bound.
 #include "qsv_fused.h"
+#include "qsv_mfma.h"
 
 namespace qsv {
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  typedef double acc __attribute__((ext_vector_type(4)));
-  __device__ static acc op(double a, double b, acc c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<float> {
-  typedef float acc __attribute__((ext_vector_type(4)));
-  __device__ static acc op(float a, float b, acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-};
 
 int fused_nb(int elem_bytes, int D, int vn) {
   if (D != 64) return 1;

@@ -6,84 +6,94 @@
 // loss with respect to the result.  (The gradient with respect to x is the
 // same contraction with the transposed PTM and reuses the apply kernels.)
 //
-// Blocks stage chunks of orbits in LDS and accumulate the outer products in
-// float64 registers; blocks write float64 partials, a second kernel sums the
-// partials in a fixed order (deterministic).
+// The sum over orbits is a (D_out x G) (G x D_in) matrix product with a
+// huge inner dimension, done on v_mfma_f64_16x16x4f64: each k-step takes
+// four orbits, lane (c, k) = (l % 16, l / 16) supplying ybar[g_k][c] as the
+// A operand and x[g_k][c] as the B operand, so one lane loads both operands
+// of its own orbit straight from HBM (float32 states are widened to float64:
+// products and sums are float64).  When the innermost axis is a bystander
+// with unit stride a lane loads VN consecutive orbits as one 16-byte vector
+// (VN k-steps); otherwise lanes c of one orbit read the targets' entries (a
+// contiguous run when the targets are the innermost axes).  Blocks write
+// float64 partial tiles, a second kernel sums them in a fixed order
+// (deterministic).
 #include "qsv_geom.h"
+#include "qsv_mfma.h"
 
 #include <string.h>
+#include <algorithm>
 #include <vector>
 
 namespace qsv {
 
 constexpr int kGradBlocks = 256;
+constexpr int kGradUnroll = 4;
 
 struct GradOffs {
   int64_t off_in[16];
   int64_t off_out[16];
 };
 
-// A block walks chunks of CHUNK orbits: thread t gathers orbit t's D_in
-// entries of x and D_out entries of ybar into LDS (orbit-major rows, so the
-// gathers of neighbouring lanes hit neighbouring orbits), then wave w
-// accumulates the outer products of its quarter of the chunk, lane l owning
-// dP[o = l / 4][4 (l % 4) .. 4 (l % 4) + 3] in float64.  x and ybar are read
-// once.  The four waves' sums meet in LDS at the end.
-template <typename T, int CHUNK>
+template <typename T, int VN>
 __global__ void __launch_bounds__(kBlock)
     k_ptm_grad(const T* __restrict__ x, const T* __restrict__ ybar, const Geom g,
                const GradOffs offs, const uint32_t n_orbits, double* __restrict__ partial) {
-  constexpr int LD = 17;  // padded row of 16
-  __shared__ T X[CHUNK * LD];
-  __shared__ T Y[CHUNK * LD];
+  typedef typename Mfma<double>::acc Acc;
+  typedef typename VT<T, VN>::t Vec;
   __shared__ double red[kBlock / 64][64 * 4];
   const uint32_t b = blockIdx.y;
   x += int64_t(b) * g.bs_in;
   ybar += int64_t(b) * g.bs_out;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int o = lane >> 2, i0 = (lane & 3) * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  constexpr int PER_WAVE = CHUNK / (kBlock / 64);
-
-  for (uint32_t c0 = blockIdx.x * uint32_t(CHUNK); c0 < n_orbits;
-       c0 += gridDim.x * uint32_t(CHUNK)) {
-    for (int j = tid; j < CHUNK; j += kBlock) {
-      const uint32_t m = c0 + j;
-      if (m < n_orbits) {
-        int64_t bin, bout;
-        orbit_base(g, m, bin, bout);
+  const int c = lane & 15, k = lane >> 4;
+  const bool vi = c < g.din, vo = c < g.dout;
+  const int64_t oi = vi ? offs.off_in[c] : 0, oo = vo ? offs.off_out[c] : 0;
+  Acc acc = Acc(0);
+  constexpr uint32_t STEP = 4 * VN;  // orbits per wave step
+  constexpr int U = kGradUnroll;     // steps whose loads are issued together
+  const uint32_t nsteps = (n_orbits + STEP - 1) / STEP;
+  const uint32_t nchunks = (nsteps + U - 1) / U;
+  for (uint32_t tc = blockIdx.x * (kBlock / 64) + w; tc < nchunks;
+       tc += gridDim.x * (kBlock / 64)) {
+    // loads of all U steps first (addresses clamped to entry 0 and values
+    // zeroed for lanes past the end: no branches between the loads)
+    Vec xv[U], yv[U];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) X[j * LD + i] = i < g.din ? x[bin + offs.off_in[i]] : T(0);
+    for (int u = 0; u < U; ++u) {
+      const uint32_t m = (tc * U + u) * STEP + VN * k;
+      const bool ok = m < n_orbits;  // VN divides n_orbits: whole vectors
+      int64_t bin, bout;
+      orbit_base(g, ok ? m : 0, bin, bout);
+      const bool lx = ok && vi, ly = ok && vo;
+      xv[u] = *reinterpret_cast<const Vec*>(x + (lx ? bin + oi : 0));
+      yv[u] = *reinterpret_cast<const Vec*>(ybar + (ly ? bout + oo : 0));
+      if (!lx) xv[u] = Vec(0);
+      if (!ly) yv[u] = Vec(0);
+    }
 #pragma unroll
-        for (int r = 0; r < 16; ++r)
-          Y[j * LD + r] = r < g.dout ? ybar[bout + offs.off_out[r]] : T(0);
+    for (int u = 0; u < U; ++u) {
+      if constexpr (VN == 1) {
+        acc = Mfma<double>::op(double(yv[u]), double(xv[u]), acc);
       } else {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) X[j * LD + i] = Y[j * LD + i] = T(0);
+        for (int e = 0; e < VN; ++e)
+          acc = Mfma<double>::op(double(yv[u][e]), double(xv[u][e]), acc);
       }
     }
-    __syncthreads();
-    const int j0 = w * PER_WAVE;
-#pragma unroll 4
-    for (int j = j0; j < j0 + PER_WAVE; ++j) {
-      const double yv = double(Y[j * LD + o]);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] += yv * double(X[j * LD + i0 + c]);
-    }
-    __syncthreads();
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) red[w][lane * 4 + c] = acc[c];
+  for (int r = 0; r < 4; ++r) red[w][lane * 4 + r] = acc[r];
   __syncthreads();
-  // thread t < 256 sums entry t = (o, i) over the four waves
+  // thread t sums entry (o, i) = (t / 16, t % 16) over the four waves; it
+  // sits in column i, row o = k + 4 r of the float64 accumulator layout
   {
-    const int oo = tid >> 4, ii = tid & 15;
-    const int src = (oo * 4 + (ii >> 2)) * 4 + (ii & 3);  // lane (oo, ii / 4), slot ii % 4
-    double s = 0.0;
+    const int o = tid >> 4, i = tid & 15;
+    const int src = (i + 16 * (o & 3)) * 4 + (o >> 2);
+    double sum = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < kBlock / 64; ++ww) s += red[ww][src];
-    if (oo < g.dout && ii < g.din)
-      partial[((int64_t(b) * gridDim.x + blockIdx.x) * g.dout + oo) * g.din + ii] = s;
+    for (int ww = 0; ww < kBlock / 64; ++ww) sum += red[ww][src];
+    if (o < g.dout && i < g.din)
+      partial[((int64_t(b) * gridDim.x + blockIdx.x) * g.dout + o) * g.din + i] = sum;
   }
 }
 
@@ -108,12 +118,13 @@ struct GradPlan {
   GradOffs offs;
   uint32_t n_orbits;
   int din, dout;
+  int vn;  // orbits per 16-byte lane vector (1: scalar loads)
   int64_t n_in, n_out;
 };
 
 // Geometry of the contraction without vector widths (one orbit per step).
-static int make_grad_plan(int rank, const int64_t* dims, int k, const int32_t* targets,
-                          const int32_t* dims_out_t, GradPlan& p) {
+static int make_grad_plan(int elem_bytes, int rank, const int64_t* dims, int k,
+                          const int32_t* targets, const int32_t* dims_out_t, GradPlan& p) {
   QSV_CHECK_ARG(rank >= 1 && rank <= QSV_MAX_RANK, "rank %d outside [1, %d]", rank,
                 QSV_MAX_RANK);
   QSV_CHECK_ARG(k >= 1 && k <= QSV_MAX_TARGETS && k <= rank, "bad number of targets %d", k);
@@ -199,11 +210,30 @@ static int make_grad_plan(int rank, const int64_t* dims, int k, const int32_t* t
   QSV_CHECK_ARG(M < (int64_t(1) << 32) - kBlock, "too many orbits");
   p.n_orbits = uint32_t(M);
   g.n_threads = p.n_orbits;
+  // lanes load 16-byte vectors of consecutive orbits when the innermost axis
+  // is a unit-stride bystander and every base, offset and batch stride keeps
+  // them aligned
+  p.vn = 1;
+  const int want = 16 / elem_bytes;
+  if (ng > 0 && g.gs_in[0] == 1 && g.gs_out[0] == 1 && g.gdiv[0].d % want == 0 &&
+      p.n_in % want == 0 && p.n_out % want == 0) {
+    bool aligned = true;
+    for (int j = 1; j < ng; ++j)
+      aligned = aligned && g.gs_in[j] % want == 0 && g.gs_out[j] % want == 0;
+    for (int i = 0; i < p.din; ++i) aligned = aligned && p.offs.off_in[i] % want == 0;
+    for (int o = 0; o < p.dout; ++o) aligned = aligned && p.offs.off_out[o] % want == 0;
+    if (aligned) p.vn = want;
+  }
   return QSV_OK;
 }
 
-static int grad_blocks(uint32_t n_orbits) {
-  const uint32_t b = (n_orbits + kBlock * 2 - 1) / (kBlock * 2);
+// blocks per state: at least 1024 orbits per block, at most kGradBlocks,
+// and no more than needed for ~4096 blocks over the whole batch (fewer
+// partial tiles for the final sum to read)
+static int grad_blocks(uint32_t n_orbits, int batch) {
+  uint32_t b = (n_orbits + 1023) / 1024;
+  const uint32_t fill = uint32_t((4096 + std::max(batch, 1) - 1) / std::max(batch, 1));
+  b = std::min(b, fill);
   return int(b < 1 ? 1 : (b > kGradBlocks ? kGradBlocks : b));
 }
 
@@ -215,37 +245,61 @@ extern "C" int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64
                                                 const int32_t* targets,
                                                 const int32_t* dims_out_t) {
   GradPlan p;
-  const int rc = make_grad_plan(rank, dims, k, targets, dims_out_t, p);
+  const int rc = make_grad_plan(8, rank, dims, k, targets, dims_out_t, p);
   if (rc) return rc;
-  return int64_t(batch) * grad_blocks(p.n_orbits) * p.din * p.dout * 8;
+  return int64_t(batch) * kGradBlocks * p.din * p.dout * 8;
+}
+
+template <typename T>
+static void launch_grad(const GradPlan& p, int vn, dim3 grid, const void* x, const void* ybar,
+                        double* partial, hipStream_t s) {
+  const T* xx = static_cast<const T*>(x);
+  const T* yy = static_cast<const T*>(ybar);
+  if (vn == 16 / int(sizeof(T)))
+    hipLaunchKernelGGL((k_ptm_grad<T, 16 / sizeof(T)>), grid, dim3(kBlock), 0, s, xx, yy, p.g,
+                       p.offs, p.n_orbits, partial);
+  else
+    hipLaunchKernelGGL((k_ptm_grad<T, 1>), grid, dim3(kBlock), 0, s, xx, yy, p.g, p.offs,
+                       p.n_orbits, partial);
 }
 
 extern "C" int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batch, int rank,
                             const int64_t* dims, int k, const int32_t* targets,
                             const int32_t* dims_out_t, void* dptm, int accumulate,
                             void* workspace, int64_t workspace_bytes, void* stream) {
+  return qsv_ptm_grad_tiled(dtype, x, ybar, batch, rank, dims, k, targets, dims_out_t, dptm,
+                            accumulate, workspace, workspace_bytes, nullptr, stream);
+}
+
+extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, int batch, int rank,
+                                  const int64_t* dims, int k, const int32_t* targets,
+                                  const int32_t* dims_out_t, void* dptm, int accumulate,
+                                  void* workspace, int64_t workspace_bytes, const qsv_tile* tile,
+                                  void* stream) {
   QSV_CHECK_ARG(dtype == QSV_F32 || dtype == QSV_F64, "unknown dtype %d", dtype);
+  QSV_CHECK_ARG(!tile || (tile->grad_blocks >= 0 && tile->grad_blocks <= kGradBlocks),
+                "tile: grad_blocks outside [0, %d]", kGradBlocks);
   QSV_CHECK_ARG(batch >= 0 && batch <= 65535, "batch %d outside [0, 65535]", batch);
   GradPlan p;
-  const int rc = make_grad_plan(rank, dims, k, targets, dims_out_t, p);
+  const int rc = make_grad_plan(dtype_size(dtype), rank, dims, k, targets, dims_out_t, p);
   if (rc) return rc;
   if (batch == 0) return QSV_OK;
   QSV_CHECK_ARG(x && ybar && dptm, "null pointer argument");
-  const int nb = grad_blocks(p.n_orbits);
+  const int nb = tile && tile->grad_blocks > 0 ? tile->grad_blocks
+                                               : grad_blocks(p.n_orbits, batch);
   const int64_t need = int64_t(batch) * nb * p.din * p.dout * 8;
   QSV_CHECK_ARG(workspace && workspace_bytes >= need, "PTM gradient needs %lld workspace bytes",
                 (long long)need);
   hipStream_t s = static_cast<hipStream_t>(stream);
   double* partial = static_cast<double*>(workspace);
   const dim3 grid(nb, batch);
+  const bool aligned = reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+                       reinterpret_cast<uintptr_t>(ybar) % 16 == 0;
+  const int vn = aligned ? p.vn : 1;
   if (dtype == QSV_F64)
-    hipLaunchKernelGGL((k_ptm_grad<double, 128>), grid, dim3(kBlock), 0, s,
-                       static_cast<const double*>(x), static_cast<const double*>(ybar), p.g,
-                       p.offs, p.n_orbits, partial);
+    launch_grad<double>(p, vn, grid, x, ybar, partial, s);
   else
-    hipLaunchKernelGGL((k_ptm_grad<float, 256>), grid, dim3(kBlock), 0, s,
-                       static_cast<const float*>(x), static_cast<const float*>(ybar), p.g,
-                       p.offs, p.n_orbits, partial);
+    launch_grad<float>(p, vn, grid, x, ybar, partial, s);
   QSV_CHECK_HIP(hipGetLastError());
   const int dd = p.din * p.dout;
   const dim3 g2((dd + kBlock - 1) / kBlock, batch);
@@ -257,5 +311,102 @@ extern "C" int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batc
     hipLaunchKernelGGL(k_ptm_grad_final<float>, g2, dim3(kBlock), 0, s, partial, nb, dd,
                        static_cast<float*>(dptm), acc_flag);
   QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+// Host replay of k_ptm_grad + k_ptm_grad_final for one state: every lane's
+// loads (bounds and vector alignment checked), the MFMA contraction in
+// hardware index order (A = ybar, B = x, k = the four orbits of a k-step),
+// the block reduction's accumulator-to-entry map and the fixed-order final
+// sum.  Every orbit must be consumed exactly once.
+extern "C" int qsv_emulate_ptm_grad(int dtype, const double* x, const double* ybar, int rank,
+                                    const int64_t* dims, int k, const int32_t* targets,
+                                    const int32_t* dims_out_t, double* dptm,
+                                    int32_t grad_blocks_forced, int32_t* vn_out) {
+  QSV_CHECK_ARG(dtype == QSV_F32 || dtype == QSV_F64, "unknown dtype %d", dtype);
+  GradPlan p;
+  const int rc = make_grad_plan(dtype_size(dtype), rank, dims, k, targets, dims_out_t, p);
+  if (rc) return rc;
+  QSV_CHECK_ARG(x && ybar && dptm, "null pointer argument");
+  QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24) && p.n_out <= (int64_t(1) << 24),
+                "emulation is for small vectors");
+  QSV_CHECK_ARG(grad_blocks_forced >= 0 && grad_blocks_forced <= kGradBlocks,
+                "grad_blocks outside [0, %d]", kGradBlocks);
+  const int VN = p.vn, waves = kBlock / 64;
+  const int nb = grad_blocks_forced > 0 ? grad_blocks_forced : grad_blocks(p.n_orbits, 1);
+  if (vn_out) *vn_out = VN;
+  const uint32_t STEP = 4 * VN, nsteps = (p.n_orbits + STEP - 1) / STEP;
+  std::vector<int> used(p.n_orbits, 0);
+  std::vector<double> partial(size_t(nb) * p.din * p.dout, 0.0);
+  for (int blk = 0; blk < nb; ++blk) {
+    // acc[w][lane][r]
+    std::vector<double> acc(size_t(waves) * 64 * 4, 0.0);
+    const uint32_t nchunks = (nsteps + kGradUnroll - 1) / kGradUnroll;
+    for (int w = 0; w < waves; ++w)
+      for (uint32_t tc = blk * waves + w; tc < nchunks; tc += nb * waves)
+      for (uint32_t t = tc * kGradUnroll; t < tc * kGradUnroll + kGradUnroll; ++t) {
+        // operands of this wave iteration: lane (c, kk) -> VN k-steps
+        double A[64][16], B[64][16];  // [lane][e]
+        for (int lane = 0; lane < 64; ++lane) {
+          const int c = lane & 15, kk = lane >> 4;
+          const uint32_t m = t * STEP + VN * kk;
+          const bool ok = m < p.n_orbits;
+          int64_t bin, bout;
+          orbit_base(p.g, ok ? m : 0, bin, bout);
+          for (int e = 0; e < VN; ++e) {
+            A[lane][e] = B[lane][e] = 0.0;
+            if (!ok) continue;
+            if (e == 0 && c == 0) {
+              for (int ee = 0; ee < VN; ++ee) {
+                QSV_CHECK_ARG(m + ee < p.n_orbits, "grad: vector past the last orbit");
+                int64_t bi2, bo2;
+                orbit_base(p.g, m + ee, bi2, bo2);
+                QSV_CHECK_ARG(bi2 == bin + ee && bo2 == bout + ee,
+                              "grad: orbits of a vector not adjacent");
+                ++used[m + ee];
+              }
+            }
+            if (c < p.din) {
+              const int64_t a = bin + p.offs.off_in[c] + e;
+              QSV_CHECK_ARG((a - e) % VN == 0, "grad: misaligned vector load");
+              QSV_CHECK_ARG(a >= 0 && a < p.n_in, "grad: x load %lld out of bounds", (long long)a);
+              B[lane][e] = x[a];
+            }
+            if (c < p.dout) {
+              const int64_t a = bout + p.offs.off_out[c] + e;
+              QSV_CHECK_ARG((a - e) % VN == 0, "grad: misaligned vector load");
+              QSV_CHECK_ARG(a >= 0 && a < p.n_out, "grad: ybar load %lld out of bounds",
+                            (long long)a);
+              A[lane][e] = ybar[a];
+            }
+          }
+        }
+        // D[row][col] += sum_kk A[row + 16 kk] B[col + 16 kk]; slot r of lane l
+        // holds row (l / 16) + 4 r, column l % 16
+        for (int e = 0; e < VN; ++e)
+          for (int lane = 0; lane < 64; ++lane)
+            for (int r = 0; r < 4; ++r) {
+              const int row = (lane >> 4) + 4 * r, col = lane & 15;
+              double d = 0.0;
+              for (int kk = 0; kk < 4; ++kk) d += A[row + 16 * kk][e] * B[col + 16 * kk][e];
+              acc[(size_t(w) * 64 + lane) * 4 + r] += d;
+            }
+      }
+    for (int tid = 0; tid < kBlock; ++tid) {
+      const int o = tid >> 4, i = tid & 15;
+      const int src = (i + 16 * (o & 3)) * 4 + (o >> 2);
+      double sum = 0.0;
+      for (int w = 0; w < waves; ++w) sum += acc[size_t(w) * 64 * 4 + src];
+      if (o < p.dout && i < p.din) partial[(size_t(blk) * p.dout + o) * p.din + i] = sum;
+    }
+  }
+  for (uint32_t m = 0; m < p.n_orbits; ++m)
+    QSV_CHECK_ARG(used[m] == 1, "grad: orbit %u consumed %d times", m, used[m]);
+  const int dd = p.din * p.dout;
+  for (int j = 0; j < dd; ++j) {
+    double s = 0.0;
+    for (int blk = 0; blk < nb; ++blk) s += partial[size_t(blk) * dd + j];
+    dptm[j] = s;
+  }
   return QSV_OK;
 }

@@ -7,6 +7,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
+# tests tune launch shapes in memory only (no cache file read or written)
+os.environ.setdefault('QSV_TUNE_CACHE', '')
 
 
 def pytest_configure(config):

@@ -145,3 +145,22 @@ def test_t1t2_rates_closed_form():
     g, lam = t1t2_rates(t, t1, t2)
     assert np.isclose(np.sqrt(1 - g) * np.sqrt(1 - lam), np.exp(-t / t2), rtol=1e-14)
     assert np.isclose(1 - g, np.exp(-t / t1), rtol=1e-14)
+
+
+def test_autotune_cache_roundtrip(tmp_path):
+    """The tuner's cache file: entries survive save/load, power-of-two batch
+    buckets, summary sums (no GPU needed)."""
+    from quantumsim_amd.autotune import Autotuner, _bucket
+    assert [_bucket(b) for b in (1, 2, 3, 4, 5, 64, 65)] == [1, 2, 4, 4, 8, 64, 128]
+    path = str(tmp_path / 'tiles.json')
+    t = Autotuner(path=path)
+    t.entries['apply|float32|64|4x4|0,1|4,4|dev'] = {'kernel': 2, 'tile_orbits': 256,
+                                                     'us': 10.0, 'default_us': 12.5}
+    t.entries['grad|float32|64|4x4|0,1|4,4|dev'] = {'grad_blocks': 32, 'us': 5.0,
+                                                    'default_us': 5.5}
+    t.save()
+    t2 = Autotuner(path=path)
+    assert t2.entries == t.entries
+    s = t2.summary()
+    assert s['apply'] == {'geometries': 1, 'default_us': 12.5, 'tuned_us': 10.0}
+    assert s['grad']['geometries'] == 1

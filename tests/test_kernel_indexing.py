@@ -144,3 +144,129 @@ def test_emulated_kernels_padded_layout(dims, targets, dtype, rng):
     odd = [3 if q < 2 else 0 for q in range(n)]         # not a multiple: scalar accesses
     got, path = emulate(dtype, pv, ptm, targets, odd, [0] * n)
     assert np.allclose(got, oracle_apply(pv, ptm, targets), rtol=1e-13, atol=1e-13), path
+
+
+def emulate_grad(dtype, x, ybar, targets, dout, grad_blocks=0):
+    """Replay of qsv_ptm_grad (one state) on the host: returns (dP, vn)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    ybar = np.ascontiguousarray(ybar, dtype=np.float64)
+    k = len(targets)
+    din = [x.shape[t] for t in targets]
+    out = np.zeros(int(np.prod(dout)) * int(np.prod(din)))
+    vn = (nat.ctypes.c_int32 * 1)()
+    rc = nat.lib().qsv_emulate_ptm_grad(dtype, x.ctypes.data_as(nat._c_f64p),
+                                        ybar.ctypes.data_as(nat._c_f64p), x.ndim,
+                                        nat.i64(x.shape), k, nat.i32(targets), nat.i32(dout),
+                                        out.ctypes.data_as(nat._c_f64p), grad_blocks, vn)
+    nat.check(rc, 'emulate_ptm_grad')
+    return out.reshape(list(dout) + din), vn[0]
+
+
+GRAD_GEOMS = [([4] * 5, (1, 2)), ([4] * 5, (3, 4)), ([4] * 5, (4, 3)), ([4] * 5, (0, 4)),
+              ([4] * 6, (2,)), ([4] * 6, (5,)), ([4] * 3, (0, 1)), ([4, 4], (0, 1)),
+              ([9, 4, 4, 9], (1, 2)), ([9, 4, 4, 3], (0,)), ([4, 9, 4], (2,)),
+              ([2] * 7, (3, 1)), ([4] * 7, (0, 1))]
+
+
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets', GRAD_GEOMS)
+def test_emulated_grad_matches_oracle(dims, targets, dtype, rng):
+    """dP[o, i] = sum over all other indices of ybar[.., o, ..] x[.., i, ..]:
+    the oracle's einsum, against the lane-by-lane replay of the MFMA
+    gradient kernel (square and rectangular PTMs)."""
+    n = len(dims)
+    din = [dims[t] for t in targets]
+    for dout in (din, [max(1, d - 1) for d in din]):
+        oshape = list(dims)
+        for t, d in zip(targets, dout):
+            oshape[t] = d
+        x = rng.standard_normal(dims)
+        ybar = rng.standard_normal(oshape)
+        got, vn = emulate_grad(dtype, x, ybar, targets, dout)
+        xl = list(range(n))
+        yl = list(range(n))
+        outl = [n + j for j in range(len(targets))] + [t for t in targets]
+        for j, t in enumerate(targets):
+            yl[t] = n + j
+        ref = np.einsum(ybar, yl, x, xl, outl)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (dims, targets, dout, vn)
+
+
+def test_grad_vector_width():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal([4] * 5)
+    # innermost axis a bystander: 16-byte orbit vectors
+    assert emulate_grad(nat.QSV_F64, x, x, (1, 2), [4, 4])[1] == 2
+    assert emulate_grad(nat.QSV_F32, x, x, (1, 2), [4, 4])[1] == 4
+    # innermost axis a target: per-orbit lanes
+    assert emulate_grad(nat.QSV_F32, x, x, (3, 4), [4, 4])[1] == 1
+
+
+def emulate_tiled(dtype, pv, ptm, targets, tile):
+    """Replay of the batched path's kernel for one state with launch shape
+    `tile`; returns (out, path) or None when the tile is infeasible."""
+    k = len(targets)
+    dims = list(pv.shape)
+    dout = list(ptm.shape[:k])
+    oshape = list(dims)
+    for t, d in zip(targets, dout):
+        oshape[t] = d
+    out = np.zeros(oshape)
+    x = np.ascontiguousarray(pv, dtype=np.float64)
+    P = np.ascontiguousarray(ptm, dtype=np.float64)
+    path = (nat.ctypes.c_int32 * 1)()
+    rc = nat.lib().qsv_emulate_apply_ptm_tiled(dtype, x.ctypes.data_as(nat._c_f64p),
+                                               out.ctypes.data_as(nat._c_f64p), len(dims),
+                                               nat.i64(dims), k, nat.i32(targets), nat.i32(dout),
+                                               P.ctypes.data_as(nat._c_f64p),
+                                               nat.ctypes.byref(tile), path)
+    if rc == -3:
+        return None
+    nat.check(rc, 'emulate_apply_ptm_tiled')
+    return out, path[0]
+
+
+TILE_CANDIDATES = [(1, 0), (2, 0), (2, 16), (2, 64), (2, 128), (2, 256), (2, 512), (2, 1024)]
+
+
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets', [([4] * 5, (3, 4)), ([4] * 5, (4, 3)), ([4] * 5, (1, 2)),
+                                          ([4] * 5, (0, 1)), ([4] * 6, (5,)), ([4] * 6, (2,)),
+                                          ([4] * 4, (0, 3)), ([4] * 3, (1, 2))])
+def test_emulated_tile_candidates_match_oracle(dims, targets, dtype, rng):
+    """Every launch shape the autotuner may pick computes the same result
+    (the replay checks each access and the exactly-once coverage)."""
+    pv = rng.standard_normal(dims)
+    din = [dims[t] for t in targets]
+    ptm = rng.standard_normal(din + din)
+    ref = oracle_apply(pv, ptm, targets)
+    feasible = 0
+    for kern, orbits in TILE_CANDIDATES:
+        r = emulate_tiled(dtype, pv, ptm, targets, nat.Tile(kern, orbits, 0, 0))
+        if r is None:
+            assert kern == 2
+            continue
+        feasible += 1
+        out, path = r
+        assert path == (2 if kern == 2 else 1)
+        assert np.allclose(out, ref, rtol=1e-13, atol=1e-13), (dims, targets, kern, orbits)
+    assert feasible >= 1
+
+
+def test_tile_validation():
+    pv = np.zeros([4] * 3)
+    ptm = np.zeros([4, 4])
+    for bad in (nat.Tile(3, 0, 0, 0), nat.Tile(0, -1, 0, 0), nat.Tile(0, 0, 300, 0),
+                nat.Tile(0, 0, 0, 1), nat.Tile(0, 5000, 0, 0)):
+        with pytest.raises(ValueError):
+            emulate_tiled(nat.QSV_F64, pv, ptm, (1,), bad)
+
+
+@pytest.mark.parametrize('blocks', [1, 2, 3, 7, 64, 256])
+def test_emulated_grad_block_counts(blocks, rng):
+    dims, targets = [4] * 6, (2, 3)
+    x = rng.standard_normal(dims)
+    y = rng.standard_normal(dims)
+    ref, _ = emulate_grad(nat.QSV_F64, x, y, targets, [4, 4])
+    got, _ = emulate_grad(nat.QSV_F64, x, y, targets, [4, 4], grad_blocks=blocks)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
