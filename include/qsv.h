@@ -50,6 +50,30 @@ typedef struct qsv_tile {
   int32_t reserved;     /* must be 0                                     */
 } qsv_tile;
 
+/* Tables of the persistent circuit kernels (device memory, built by the
+ * caller).  A circuit: `n` qubits in the general(2) basis, gates
+ * gates[gate0 .. gate0 + n_gates), trajectory element offset `traj` (room
+ * for n_gates states of 4^n entries: the state before each gate),
+ * population offset `pop` (2^n float64 entries).  A gate: k = 1 or 2
+ * targets q0 (, q1) in PTM axis order, PTM at element offset `ptm` of the
+ * PTM pool (4^k x 4^k, row-major, layout (out..., in...)); every gate owns
+ * its PTM slot (the adjoint writes dL/dP there). */
+typedef struct qsv_circuit {
+  int32_t n;
+  int32_t n_gates;
+  int64_t gate0;
+  int64_t traj;
+  int64_t pop;
+} qsv_circuit;
+
+typedef struct qsv_gate {
+  int32_t k;
+  int32_t q0;
+  int32_t q1;
+  int32_t reserved;
+  int64_t ptm;
+} qsv_gate;
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* qsv_last_error(void);
 
@@ -118,6 +142,34 @@ int qsv_apply_ptm_batched_tiled(int dtype, const void* in, void* out, int batch,
                                 const int64_t* dims, int k, const int32_t* targets,
                                 const int32_t* dims_out_t, const void* ptm_dev,
                                 int64_t ptm_batch_stride, const qsv_tile* tile, void* stream);
+
+/* Largest qubit count whose state (and the adjoint's two states) fit the
+ * LDS of one workgroup on the current device for the persistent circuit
+ * kernels below. */
+int qsv_circuits_max_qubits(int dtype);
+
+/* LDS bytes per workgroup of the persistent forward (adjoint = 0) or
+ * adjoint (1) kernel for circuits of up to max_qubits qubits. */
+int64_t qsv_circuits_lds_bytes(int dtype, int max_qubits, int adjoint);
+
+/* One persistent launch runs every circuit of the table (widths up to
+ * max_qubits <= qsv_circuits_max_qubits, any depths): each workgroup keeps a
+ * circuit's Pauli vector in LDS, starts from |0..0><0..0|, applies the
+ * gates with the pool's PTMs, writes the state before every gate to `traj`
+ * and the float64 populations to `pops`.  Safe under HIP graph capture (one
+ * kernel, tables and buffers in device memory).  No counterpart in the
+ * reference (one circuit per PauliVector object there); used by the circuit
+ * batcher for its narrow circuits. */
+int qsv_circuits_forward(int dtype, int n_circuits, const qsv_circuit* circs_dev,
+                         const qsv_gate* gates_dev, const void* ptms, void* traj,
+                         double* pops, int max_qubits, void* stream);
+
+/* Adjoint of qsv_circuits_forward: from dL/dpops (float64) and the forward
+ * trajectory, writes dL/dP of every gate (float64, fixed summation order)
+ * into dptms at the gate's PTM offset.  One persistent launch. */
+int qsv_circuits_adjoint(int dtype, int n_circuits, const qsv_circuit* circs_dev,
+                         const qsv_gate* gates_dev, const void* ptms, const void* traj,
+                         const double* dpops, double* dptms, int max_qubits, void* stream);
 
 /* Bytes of device workspace qsv_ptm_grad / qsv_ptm_grad_tiled need (enough
  * for any launch shape). */
@@ -224,6 +276,16 @@ int qsv_emulate_apply_ptm(int dtype, const double* in, double* out, int rank,
                           const int64_t* out_strides, int k, const int32_t* targets,
                           const int32_t* dims_out_t, const double* ptm_f64,
                           int32_t* path_out);
+
+/* CPU replay of qsv_circuits_forward (and, with dpops != NULL, of
+ * qsv_circuits_adjoint) on host tables and float64 host arrays, checking
+ * every table entry against the array sizes, every orbit's LDS indices and
+ * that each gate touches every state entry exactly once.  No counterpart in
+ * the reference; it pins the table builder and the kernels' indexing. */
+int qsv_emulate_circuits(int n_circuits, const qsv_circuit* circs, const qsv_gate* gates,
+                         int64_t n_gates_total, const double* ptms, int64_t ptm_elems,
+                         double* traj, int64_t traj_elems, double* pops, int64_t pop_elems,
+                         const double* dpops, double* dptms, int max_qubits);
 
 /* qsv_emulate_apply_ptm of the batched path's kernel choice for one state,
  * with an explicit launch shape (the autotuner's candidates are replayed

@@ -68,6 +68,15 @@ _SIGNATURES = {
     'qsv_emulate_ptm_grad': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, _c_i64p,
                                             ctypes.c_int, _c_i32p, _c_i32p, _c_f64p,
                                             ctypes.c_int32, _c_i32p]),
+    'qsv_circuits_max_qubits': (ctypes.c_int, [ctypes.c_int]),
+    'qsv_circuits_lds_bytes': (ctypes.c_int64, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    'qsv_circuits_forward': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                            ctypes.c_int, _vp]),
+    'qsv_circuits_adjoint': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                            _vp, ctypes.c_int, _vp]),
+    'qsv_emulate_circuits': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int64, _c_f64p,
+                                            ctypes.c_int64, _c_f64p, ctypes.c_int64, _c_f64p,
+                                            ctypes.c_int64, _c_f64p, _c_f64p, ctypes.c_int]),
     'qsv_ptm_grad_tiled': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                           _c_i64p, ctypes.c_int, _c_i32p, _c_i32p, _vp,
                                           ctypes.c_int, _vp, ctypes.c_int64, _vp, _vp]),

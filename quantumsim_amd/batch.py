@@ -1,14 +1,23 @@
 """Many noisy circuits of different widths and depths, simulated together.
 
-Circuits are grouped by qubit count; inside a group the states form one
-[B, 4^n] tensor ordered by decreasing depth, so the circuits still running
-at any layer are a prefix of the batch.  Every fused gate of a layer is one
-qsv_apply_ptm_batched launch over that prefix (per-circuit PTMs built on the
-device from the circuit parameters); circuits that have finished are
-retired by narrowing the prefix, without copies.  Forward, the float64
-population readout and the adjoint (checkpointed recompute + qsv_ptm_grad)
-are all device work with no host synchronisation, so the whole step can be
-captured once in a HIP graph and replayed (``CircuitBatch.capture``).
+Narrow circuits (up to ``persistent.max_qubits()`` qubits, 7 in float32 on
+MI355X) all run in ONE persistent kernel launch (forward) and one more
+(adjoint): a workgroup per circuit, its Pauli vector in LDS, every width and
+depth in the same launch (quantumsim_amd/persistent.py).  The PTMs of all
+their gates are computed by torch from the circuit parameters in a handful
+of batched ops (all layers of a parity at once) into one pool, so autograd
+carries dL/dP from the adjoint kernel to the angles and T1/T2.
+
+Wider circuits (whose states, 256 KB to 16 MB each, do not fit in LDS and
+are streamed at HBM bandwidth anyway) are grouped by qubit count; inside a
+group the states form one [B, 4^n] tensor ordered by decreasing depth, so
+the circuits still running at any layer are a prefix of the batch.  Every
+fused gate of a layer is one qsv_apply_ptm_batched launch over that prefix;
+circuits that have finished are retired by narrowing the prefix, without
+copies.  Forward, the float64 population readout and the adjoint
+(checkpointed recompute + qsv_ptm_grad) are all device work with no host
+synchronisation, so the whole step can be captured once in a HIP graph and
+replayed (``CircuitBatch.capture``).
 
 The reference simulates one circuit per PauliVector object; this module has
 no counterpart there.  Its per-gate arithmetic is the reference's
@@ -19,10 +28,11 @@ import math
 import numpy as np
 import torch
 
+from . import persistent
 from .autograd import run_circuit, diagonal
 from .autotune import tuning
 from .bases import general
-from .workloads import rotation_ptms, damping_ptms, fused_layer_gates
+from .workloads import rotation_ptms, damping_ptms, fused_layer_gates, layer_targets
 
 __all__ = ['CircuitBatch', 'plain_forward_backward']
 
@@ -54,7 +64,7 @@ class CircuitBatch:
     gradients with respect to every parameter."""
 
     def __init__(self, circuits, dtype=torch.float32, device=None, checkpoint_every=16,
-                 requires_grad=True):
+                 requires_grad=True, persistent_kernel=True):
         self.device = device or torch.device('cuda', torch.cuda.current_device())
         self.dtype = dtype
         self.k = checkpoint_every
@@ -75,6 +85,71 @@ class CircuitBatch:
                     t.requires_grad_(requires_grad)
                 self.groups.append(g)
         self.graph = None
+        # narrow circuits: one persistent launch (tables built once here)
+        self.lds_qubits = persistent.max_qubits(dtype) if persistent_kernel else 0
+        self.small = [gi for gi, g in enumerate(self.groups) if g.n <= self.lds_qubits]
+        self.ptable = self._persistent_table() if self.small else None
+
+    def _pool_layout(self):
+        """(group index, parity, gate index within the layer, D, layers of
+        that parity) of every piece of the PTM pool, in pool order."""
+        out = []
+        for gi in self.small:
+            g = self.groups[gi]
+            for p in (0, 1):
+                Lp = len(range(p, g.layers[0], 2))
+                if Lp == 0:
+                    continue
+                for j, t in enumerate(layer_targets(g.n, p)):
+                    out.append((gi, p, j, 4 ** len(t), Lp))
+        return out
+
+    def _persistent_table(self):
+        base, off = {}, 0
+        for gi, p, j, D, Lp in self._pool_layout():
+            base[(gi, p, j)] = (off, Lp)
+            off += len(self.groups[gi].members) * Lp * D * D
+        circuits = []
+        for gi in self.small:
+            g = self.groups[gi]
+            targets = [layer_targets(g.n, p) for p in (0, 1)]
+            for b, L in enumerate(g.layers):
+                gates = []
+                for l in range(L):
+                    p, lp = l % 2, l // 2
+                    for j, t in enumerate(targets[p]):
+                        b0, Lp = base[(gi, p, j)]
+                        D = 4 ** len(t)
+                        gates.append((t, b0 + (b * Lp + lp) * D * D))
+                circuits.append((g.n, gates))
+        return persistent.CircuitTable(circuits, off, device=self.device)
+
+    def _persistent_populations(self):
+        """Populations of every narrow circuit: one forward launch over all
+        of them; per group [B, 2^n] in member order."""
+        pieces = []
+        for gi in self.small:
+            g = self.groups[gi]
+            B, n = len(g.members), g.n
+            rot = rotation_ptms(g.angles)            # [B, L, n, 4, 4]
+            damp = damping_ptms(g.t1, g.t2)          # [B, n, 4, 4]
+            for p in (0, 1):
+                Lp = len(range(p, g.layers[0], 2))
+                if Lp == 0:
+                    continue
+                rot_p = rot[:, p::2].reshape(B * Lp, n, 4, 4)
+                damp_p = damp[:, None].expand(B, Lp, n, 4, 4).reshape(B * Lp, n, 4, 4)
+                for gate in fused_layer_gates(n, p, rot_p, damp_p):
+                    pieces.append(gate.ptm.reshape(-1))
+        pool = torch.cat(pieces)
+        pops = persistent.run(pool, self.ptable, self.dtype)
+        out, pos = [], 0
+        for gi in self.small:
+            g = self.groups[gi]
+            size = len(g.members) * 2 ** g.n
+            out.append(pops[pos:pos + size].reshape(len(g.members), 2 ** g.n))
+            pos += size
+        return out
 
     def parameters(self):
         return [t for g in self.groups for t in (g.angles, g.t1, g.t2)]
@@ -110,7 +185,14 @@ class CircuitBatch:
 
     def populations(self):
         """float64 populations per group ([B_n, 2^n] tensors, depth-sorted)."""
-        return [self._group_populations(g) for g in self.groups]
+        out = [None] * len(self.groups)
+        if self.small:
+            for gi, p in zip(self.small, self._persistent_populations()):
+                out[gi] = p
+        for gi, g in enumerate(self.groups):
+            if out[gi] is None:
+                out[gi] = self._group_populations(g)
+        return out
 
     def _step(self, weights):
         pops = self.populations()
@@ -164,5 +246,6 @@ def plain_forward_backward(circuit, weight, dtype=torch.float32, device=None,
     """One circuit on its own (batch of 1, eager launches): the per-circuit
     baseline the batched engine is compared with."""
     device = device or torch.device('cuda', torch.cuda.current_device())
-    b = CircuitBatch([circuit], dtype=dtype, device=device, checkpoint_every=checkpoint_every)
+    b = CircuitBatch([circuit], dtype=dtype, device=device, checkpoint_every=checkpoint_every,
+                     persistent_kernel=False)
     return b.step([weight])

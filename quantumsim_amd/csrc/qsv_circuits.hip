@@ -1,0 +1,416 @@
+// Persistent kernels that run whole circuits: many qubit circuits of
+// different widths and depths in one launch, the Pauli vector of each
+// circuit resident in LDS (4^n entries: up to 7 qubits in float32 on
+// gfx950's 160 KB of LDS per workgroup for the forward pass).
+//
+// A workgroup takes circuits from the circuit table (round robin over the
+// grid) and, for each, runs its gates from the gate table:
+//
+//   forward:  S = |0..0><0..0|; per gate: save S to the trajectory (the
+//             state before the gate), stage the PTM in LDS, contract every
+//             orbit in place (an orbit's D entries are read into registers,
+//             then overwritten; orbits are disjoint, so no other thread
+//             touches them); at the end write the float64 populations
+//             (the entries whose qubit digits are all |0><0| or |1><1|).
+//   adjoint:  Y = dL/dS (the population gradient scattered onto those
+//             entries); per gate, last first: load the state before the gate
+//             from the trajectory, dL/dP[o][i] = sum over orbits of
+//             Y[o] X[i] (float64, fixed order: threads own (o, i) entries and
+//             slices of the orbits, slices summed in order), then Y <- P^T Y
+//             per orbit in place.
+//
+// All qubits are in the general(2) basis (Pauli dimension 4: two bits per
+// digit), qubit 0 most significant.  Gates act on 1 or 2 qubits with PTMs
+// of 4 x 4 or 16 x 16, layout (out..., in...) in target order, as
+// qsv_apply_ptm.  No counterpart in the reference (one circuit per
+// PauliVector object there); the per-gate arithmetic is its apply_ptm
+// (quantumsim/pauli_vectors/numpy.py:50-64).
+#include "qsv_common.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace qsv {
+
+constexpr int kCircBlock = 256;
+constexpr int kCircMaxQubits = 8;
+
+// Index of orbit m with zero 2-bit digits inserted at bit positions lo < hi
+// (hi < 0: one target).
+__host__ __device__ inline uint32_t circ_orbit_base(uint32_t m, int lo, int hi) {
+  uint32_t t = ((m >> lo) << (lo + 2)) | (m & ((1u << lo) - 1u));
+  if (hi >= 0) t = ((t >> hi) << (hi + 2)) | (t & ((1u << hi) - 1u));
+  return t;
+}
+
+// Offset of PTM index i (target order: first target slowest) inside an orbit.
+__host__ __device__ inline uint32_t circ_offset(int k, uint32_t i, int s0, int s1) {
+  return k == 1 ? (i << s0) : (((i >> 2) << s0) | ((i & 3u) << s1));
+}
+
+// Pauli index of population j (qubit 0 = most significant bit of j): bit t
+// of j becomes bit 2 t.
+__host__ __device__ inline uint32_t circ_spread(uint32_t j) {
+  uint32_t r = 0;
+  for (int t = 0; t < 16; ++t) r |= ((j >> t) & 1u) << (2 * t);
+  return r;
+}
+
+struct GateGeom {
+  int k, D, s0, s1, lo, hi;
+  uint32_t n_orbits;
+};
+
+__host__ __device__ inline GateGeom circ_gate_geom(const qsv_gate& gd, int n) {
+  GateGeom G;
+  G.k = gd.k;
+  G.D = gd.k == 1 ? 4 : 16;
+  G.s0 = 2 * (n - 1 - gd.q0);
+  G.s1 = gd.k == 2 ? 2 * (n - 1 - gd.q1) : 0;
+  if (gd.k == 1) {
+    G.lo = G.s0;
+    G.hi = -1;
+  } else {
+    G.lo = G.s0 < G.s1 ? G.s0 : G.s1;
+    G.hi = G.s0 < G.s1 ? G.s1 : G.s0;
+  }
+  G.n_orbits = (1u << (2 * n)) / uint32_t(G.D);
+  return G;
+}
+
+// In-place contraction of every orbit of one gate (D fixed at compile time
+// so the orbit's operands stay in registers).  TRANS: Y <- P^T Y.
+template <typename T, int D, bool TRANS>
+__device__ inline void circ_orbits(T* S, const T* P, const GateGeom& G, int tid) {
+  for (uint32_t m = tid; m < G.n_orbits; m += kCircBlock) {
+    const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
+    T xin[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) xin[i] = S[base + circ_offset(G.k, i, G.s0, G.s1)];
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+      T acc = T(0);
+#pragma unroll
+      for (int i = 0; i < D; ++i) acc += (TRANS ? P[i * D + o] : P[o * D + i]) * xin[i];
+      S[base + circ_offset(G.k, o, G.s0, G.s1)] = acc;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCircBlock)
+    k_circuits_fwd(const qsv_circuit* __restrict__ circs, int n_circ,
+                   const qsv_gate* __restrict__ gates, const T* __restrict__ ptms,
+                   T* __restrict__ traj, double* __restrict__ pops, int max_qubits) {
+  extern __shared__ __align__(16) unsigned char qsv_circ_smem[];
+  T* S = reinterpret_cast<T*>(qsv_circ_smem);
+  T* P = S + (1u << (2 * max_qubits));
+  const int tid = threadIdx.x;
+  for (int c = blockIdx.x; c < n_circ; c += gridDim.x) {
+    const qsv_circuit cd = circs[c];
+    const int n = cd.n;
+    const uint32_t N = 1u << (2 * n);
+    for (uint32_t i = tid; i < N; i += kCircBlock) S[i] = i == 0 ? T(1) : T(0);
+    __syncthreads();
+    for (int g = 0; g < cd.n_gates; ++g) {
+      const qsv_gate gd = gates[cd.gate0 + g];
+      const GateGeom G = circ_gate_geom(gd, n);
+      T* tr = traj + cd.traj + int64_t(g) * N;
+      for (uint32_t i = tid; i < N; i += kCircBlock) tr[i] = S[i];
+      for (int i = tid; i < G.D * G.D; i += kCircBlock) P[i] = ptms[gd.ptm + i];
+      __syncthreads();
+      if (G.k == 1)
+        circ_orbits<T, 4, false>(S, P, G, tid);
+      else
+        circ_orbits<T, 16, false>(S, P, G, tid);
+      __syncthreads();
+    }
+    for (uint32_t j = tid; j < (1u << n); j += kCircBlock)
+      pops[cd.pop + j] = double(S[circ_spread(j)]);
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kCircBlock)
+    k_circuits_adj(const qsv_circuit* __restrict__ circs, int n_circ,
+                   const qsv_gate* __restrict__ gates, const T* __restrict__ ptms,
+                   const T* __restrict__ traj, const double* __restrict__ dpops,
+                   double* __restrict__ dptms, int max_qubits) {
+  extern __shared__ __align__(16) unsigned char qsv_circ_smem[];
+  double* red = reinterpret_cast<double*>(qsv_circ_smem);
+  T* Y = reinterpret_cast<T*>(red + kCircBlock);
+  T* X = Y + (1u << (2 * max_qubits));
+  T* P = X + (1u << (2 * max_qubits));
+  const int tid = threadIdx.x;
+  for (int c = blockIdx.x; c < n_circ; c += gridDim.x) {
+    const qsv_circuit cd = circs[c];
+    const int n = cd.n;
+    const uint32_t N = 1u << (2 * n);
+    for (uint32_t i = tid; i < N; i += kCircBlock) Y[i] = T(0);
+    __syncthreads();
+    for (uint32_t j = tid; j < (1u << n); j += kCircBlock) Y[circ_spread(j)] = T(dpops[cd.pop + j]);
+    __syncthreads();
+    for (int g = cd.n_gates - 1; g >= 0; --g) {
+      const qsv_gate gd = gates[cd.gate0 + g];
+      const GateGeom G = circ_gate_geom(gd, n);
+      const T* tr = traj + cd.traj + int64_t(g) * N;
+      for (uint32_t i = tid; i < N; i += kCircBlock) X[i] = tr[i];
+      for (int i = tid; i < G.D * G.D; i += kCircBlock) P[i] = ptms[gd.ptm + i];
+      __syncthreads();
+      // dL/dP: thread = (entry e, orbit slice)
+      const int E = G.D * G.D, slices = kCircBlock / E;
+      {
+        const int e = tid % E, sl = tid / E;
+        const uint32_t oo = circ_offset(G.k, e / G.D, G.s0, G.s1);
+        const uint32_t io = circ_offset(G.k, e % G.D, G.s0, G.s1);
+        double acc = 0.0;
+        for (uint32_t m = sl; m < G.n_orbits; m += slices) {
+          const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
+          acc += double(Y[base + oo]) * double(X[base + io]);
+        }
+        red[tid] = acc;
+      }
+      __syncthreads();
+      if (tid < E) {
+        double sum = 0.0;
+        for (int sl = 0; sl < slices; ++sl) sum += red[sl * E + tid];
+        dptms[gd.ptm + tid] = sum;
+      }
+      // Y <- P^T Y, orbit by orbit in place
+      if (G.k == 1)
+        circ_orbits<T, 4, true>(Y, P, G, tid);
+      else
+        circ_orbits<T, 16, true>(Y, P, G, tid);
+      __syncthreads();
+    }
+  }
+}
+
+static size_t circ_lds_bytes(int elem_bytes, int max_qubits, bool adjoint) {
+  const size_t N = size_t(1) << (2 * max_qubits);
+  if (!adjoint) return (N + 256) * elem_bytes;
+  return kCircBlock * 8 + (2 * N + 256) * elem_bytes;
+}
+
+static int lds_limit() {
+  static int lim = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 65536;
+    (void)hipGetLastError();
+    return v;
+  }();
+  return lim;
+}
+
+template <typename T>
+static uint32_t circ_grid(const void* kern, size_t lds, int n_circ) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, kCircBlock, lds) != hipSuccess ||
+      per <= 0)
+    per = 1;
+  (void)hipGetLastError();
+  const uint32_t cap = uint32_t(per) * uint32_t(cus);
+  return std::min<uint32_t>(cap, uint32_t(n_circ));
+}
+
+}  // namespace qsv
+
+using namespace qsv;
+
+extern "C" int qsv_circuits_max_qubits(int dtype) {
+  const int es = dtype_size(dtype);
+  QSV_CHECK_ARG(es > 0, "unknown dtype %d", dtype);
+  int best = 0;
+  for (int n = 1; n <= kCircMaxQubits; ++n)
+    if (circ_lds_bytes(es, n, true) <= size_t(lds_limit())) best = n;
+  return best;
+}
+
+extern "C" int64_t qsv_circuits_lds_bytes(int dtype, int max_qubits, int adjoint) {
+  const int es = dtype_size(dtype);
+  QSV_CHECK_ARG(es > 0, "unknown dtype %d", dtype);
+  QSV_CHECK_ARG(max_qubits >= 1 && max_qubits <= kCircMaxQubits, "max_qubits %d outside [1, %d]",
+                max_qubits, kCircMaxQubits);
+  return int64_t(circ_lds_bytes(es, max_qubits, adjoint != 0));
+}
+
+template <typename T, typename K, typename... Args>
+static int launch_circ(K kern, size_t lds, int n_circ, hipStream_t s, Args... args) {
+  static size_t attr = 0;
+  if (lds > 65536 && lds > attr) {
+    QSV_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    attr = lds;
+  }
+  const uint32_t grid = circ_grid<T>(reinterpret_cast<const void*>(kern), lds, n_circ);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kCircBlock), lds, s, args...);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+extern "C" int qsv_circuits_forward(int dtype, int n_circuits, const qsv_circuit* circs_dev,
+                                    const qsv_gate* gates_dev, const void* ptms, void* traj,
+                                    double* pops, int max_qubits, void* stream) {
+  const int es = dtype_size(dtype);
+  QSV_CHECK_ARG(es > 0, "unknown dtype %d", dtype);
+  QSV_CHECK_ARG(n_circuits >= 0, "negative circuit count");
+  if (n_circuits == 0) return QSV_OK;
+  QSV_CHECK_ARG(circs_dev && gates_dev && ptms && traj && pops, "null pointer argument");
+  QSV_CHECK_ARG(max_qubits >= 1 && max_qubits <= qsv_circuits_max_qubits(dtype),
+                "max_qubits %d outside [1, %d] (LDS per workgroup)", max_qubits,
+                qsv_circuits_max_qubits(dtype));
+  const size_t lds = circ_lds_bytes(es, max_qubits, false);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == QSV_F64)
+    return launch_circ<double>(k_circuits_fwd<double>, lds, n_circuits, s, circs_dev, n_circuits,
+                               gates_dev, static_cast<const double*>(ptms),
+                               static_cast<double*>(traj), pops, max_qubits);
+  return launch_circ<float>(k_circuits_fwd<float>, lds, n_circuits, s, circs_dev, n_circuits,
+                            gates_dev, static_cast<const float*>(ptms), static_cast<float*>(traj),
+                            pops, max_qubits);
+}
+
+extern "C" int qsv_circuits_adjoint(int dtype, int n_circuits, const qsv_circuit* circs_dev,
+                                    const qsv_gate* gates_dev, const void* ptms,
+                                    const void* traj, const double* dpops, double* dptms,
+                                    int max_qubits, void* stream) {
+  const int es = dtype_size(dtype);
+  QSV_CHECK_ARG(es > 0, "unknown dtype %d", dtype);
+  QSV_CHECK_ARG(n_circuits >= 0, "negative circuit count");
+  if (n_circuits == 0) return QSV_OK;
+  QSV_CHECK_ARG(circs_dev && gates_dev && ptms && traj && dpops && dptms, "null pointer argument");
+  QSV_CHECK_ARG(max_qubits >= 1 && max_qubits <= qsv_circuits_max_qubits(dtype),
+                "max_qubits %d outside [1, %d] (LDS per workgroup)", max_qubits,
+                qsv_circuits_max_qubits(dtype));
+  const size_t lds = circ_lds_bytes(es, max_qubits, true);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == QSV_F64)
+    return launch_circ<double>(k_circuits_adj<double>, lds, n_circuits, s, circs_dev, n_circuits,
+                               gates_dev, static_cast<const double*>(ptms),
+                               static_cast<const double*>(traj), dpops, dptms, max_qubits);
+  return launch_circ<float>(k_circuits_adj<float>, lds, n_circuits, s, circs_dev, n_circuits,
+                            gates_dev, static_cast<const float*>(ptms),
+                            static_cast<const float*>(traj), dpops, dptms, max_qubits);
+}
+
+// Host replay of both kernels on host tables and arrays (float64), with
+// every table entry, trajectory, population, PTM and LDS access checked.
+extern "C" int qsv_emulate_circuits(int n_circuits, const qsv_circuit* circs,
+                                    const qsv_gate* gates, int64_t n_gates_total,
+                                    const double* ptms, int64_t ptm_elems, double* traj,
+                                    int64_t traj_elems, double* pops, int64_t pop_elems,
+                                    const double* dpops, double* dptms, int max_qubits) {
+  QSV_CHECK_ARG(n_circuits >= 0 && circs && gates && ptms && traj && pops, "null argument");
+  QSV_CHECK_ARG(max_qubits >= 1 && max_qubits <= kCircMaxQubits, "bad max_qubits");
+  const uint32_t Nmax = 1u << (2 * max_qubits);
+  std::vector<double> S(Nmax), Y(Nmax), X(Nmax);
+  {
+    // every gate owns its PTM slot (the adjoint writes dL/dP there)
+    std::vector<std::pair<int64_t, int64_t>> slots;
+    for (int c = 0; c < n_circuits; ++c)
+      for (int g = 0; g < circs[c].n_gates && circs[c].gate0 + g < n_gates_total; ++g) {
+        const qsv_gate gd = gates[circs[c].gate0 + g];
+        slots.push_back({gd.ptm, gd.ptm + (gd.k == 1 ? 16 : 256)});
+      }
+    std::sort(slots.begin(), slots.end());
+    for (size_t i = 1; i < slots.size(); ++i)
+      QSV_CHECK_ARG(slots[i].first >= slots[i - 1].second, "two gates share PTM elements");
+  }
+  for (int c = 0; c < n_circuits; ++c) {
+    const qsv_circuit cd = circs[c];
+    QSV_CHECK_ARG(cd.n >= 1 && cd.n <= max_qubits, "circuit %d: %d qubits", c, cd.n);
+    QSV_CHECK_ARG(cd.n_gates >= 0 && cd.gate0 >= 0 && cd.gate0 + cd.n_gates <= n_gates_total,
+                  "circuit %d: gate range out of the table", c);
+    const uint32_t N = 1u << (2 * cd.n);
+    QSV_CHECK_ARG(cd.traj >= 0 && cd.traj + int64_t(cd.n_gates) * N <= traj_elems,
+                  "circuit %d: trajectory out of range", c);
+    QSV_CHECK_ARG(cd.pop >= 0 && cd.pop + (int64_t(1) << cd.n) <= pop_elems,
+                  "circuit %d: populations out of range", c);
+    for (int g = 0; g < cd.n_gates; ++g) {
+      const qsv_gate gd = gates[cd.gate0 + g];
+      QSV_CHECK_ARG(gd.k == 1 || gd.k == 2, "circuit %d gate %d: k = %d", c, g, gd.k);
+      QSV_CHECK_ARG(gd.q0 >= 0 && gd.q0 < cd.n && (gd.k == 1 || (gd.q1 >= 0 && gd.q1 < cd.n &&
+                                                                  gd.q1 != gd.q0)),
+                    "circuit %d gate %d: bad targets", c, g);
+      const int D = gd.k == 1 ? 4 : 16;
+      QSV_CHECK_ARG(gd.ptm >= 0 && gd.ptm + D * D <= ptm_elems, "circuit %d gate %d: PTM offset",
+                    c, g);
+    }
+    // forward
+    std::fill(S.begin(), S.end(), 0.0);
+    S[0] = 1.0;
+    for (int g = 0; g < cd.n_gates; ++g) {
+      const qsv_gate gd = gates[cd.gate0 + g];
+      const GateGeom G = circ_gate_geom(gd, cd.n);
+      for (uint32_t i = 0; i < N; ++i) traj[cd.traj + int64_t(g) * N + i] = S[i];
+      std::vector<int> touched(N, 0);
+      for (uint32_t m = 0; m < G.n_orbits; ++m) {
+        const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
+        double xin[16];
+        for (int i = 0; i < G.D; ++i) {
+          const uint32_t a = base + circ_offset(G.k, i, G.s0, G.s1);
+          QSV_CHECK_ARG(a < N, "circuit %d gate %d: LDS index %u out of range", c, g, a);
+          xin[i] = S[a];
+          ++touched[a];
+        }
+        for (int o = 0; o < G.D; ++o) {
+          double acc = 0.0;
+          for (int i = 0; i < G.D; ++i) acc += ptms[gd.ptm + o * G.D + i] * xin[i];
+          S[base + circ_offset(G.k, o, G.s0, G.s1)] = acc;
+        }
+      }
+      for (uint32_t a = 0; a < N; ++a)
+        QSV_CHECK_ARG(touched[a] == 1, "circuit %d gate %d: entry %u in %d orbits", c, g, a,
+                      touched[a]);
+    }
+    for (uint32_t j = 0; j < (1u << cd.n); ++j) {
+      const uint32_t a = circ_spread(j);
+      QSV_CHECK_ARG(a < N, "population index out of range");
+      pops[cd.pop + j] = S[a];
+    }
+    if (!dpops) continue;
+    QSV_CHECK_ARG(dptms != nullptr, "dptms missing");
+    // adjoint
+    std::fill(Y.begin(), Y.end(), 0.0);
+    for (uint32_t j = 0; j < (1u << cd.n); ++j) Y[circ_spread(j)] = dpops[cd.pop + j];
+    for (int g = cd.n_gates - 1; g >= 0; --g) {
+      const qsv_gate gd = gates[cd.gate0 + g];
+      const GateGeom G = circ_gate_geom(gd, cd.n);
+      for (uint32_t i = 0; i < N; ++i) X[i] = traj[cd.traj + int64_t(g) * N + i];
+      const int E = G.D * G.D, slices = kCircBlock / E;
+      std::vector<double> red(kCircBlock, 0.0);
+      for (int tid = 0; tid < kCircBlock; ++tid) {
+        const int e = tid % E, sl = tid / E;
+        const uint32_t oo = circ_offset(G.k, e / G.D, G.s0, G.s1);
+        const uint32_t io = circ_offset(G.k, e % G.D, G.s0, G.s1);
+        for (uint32_t m = sl; m < G.n_orbits; m += slices) {
+          const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
+          red[tid] += Y[base + oo] * X[base + io];
+        }
+      }
+      for (int e = 0; e < E; ++e) {
+        double sum = 0.0;
+        for (int sl = 0; sl < slices; ++sl) sum += red[sl * E + e];
+        dptms[gd.ptm + e] = sum;
+      }
+      for (uint32_t m = 0; m < G.n_orbits; ++m) {
+        const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
+        double yin[16];
+        for (int o = 0; o < G.D; ++o) yin[o] = Y[base + circ_offset(G.k, o, G.s0, G.s1)];
+        for (int i = 0; i < G.D; ++i) {
+          double acc = 0.0;
+          for (int o = 0; o < G.D; ++o) acc += ptms[gd.ptm + o * G.D + i] * yin[o];
+          Y[base + circ_offset(G.k, i, G.s0, G.s1)] = acc;
+        }
+      }
+    }
+  }
+  return QSV_OK;
+}

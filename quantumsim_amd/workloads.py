@@ -142,21 +142,28 @@ def brickwork_pairs(n, layer):
     return [(q, q + 1) for q in range(layer % 2, n - 1, 2)]
 
 
-def fused_layer_gates(n, layer, rot, damp):
-    """Gates of one layer for a batch: rot, damp [B, n, 4, 4] PTMs."""
-    import torch
-    from .autograd import Gate, cz_ptm, kron_ptm
+def layer_targets(n, layer):
+    """Targets of the fused gates of one layer, in fused_layer_gates order:
+    the brickwork bonds, then every qubit left out of a bond."""
     pairs = brickwork_pairs(n, layer)
     paired = set(q for pq in pairs for q in pq)
+    return [tuple(pq) for pq in pairs] + [(q,) for q in range(n) if q not in paired]
+
+
+def fused_layer_gates(n, layer, rot, damp):
+    """Gates of one layer for a batch: rot, damp [B, n, 4, 4] PTMs."""
+    from .autograd import Gate, cz_ptm, kron_ptm
     cz = cz_ptm(rot.device).to(rot.dtype)
     gates = []
-    for q, q2 in pairs:
-        pre = kron_ptm(rot[:, q], rot[:, q2])
-        post = kron_ptm(damp[:, q], damp[:, q2])
-        gates.append(Gate(post @ cz @ pre, (q, q2)))
-    for q in range(n):
-        if q not in paired:
-            gates.append(Gate(damp[:, q] @ rot[:, q], (q,)))
+    for t in layer_targets(n, layer):
+        if len(t) == 2:
+            q, q2 = t
+            pre = kron_ptm(rot[:, q], rot[:, q2])
+            post = kron_ptm(damp[:, q], damp[:, q2])
+            gates.append(Gate(post @ cz @ pre, t))
+        else:
+            q = t[0]
+            gates.append(Gate(damp[:, q] @ rot[:, q], t))
     return gates
 
 

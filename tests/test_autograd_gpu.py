@@ -220,3 +220,25 @@ def test_circuit_batch_float32_and_graph_replay():
             assert torch.allclose(a, b, rtol=1e-12, atol=1e-14), rep
         for a, b in zip(grads_g, grads_e):
             assert torch.allclose(a, b, rtol=1e-6, atol=1e-9), rep
+
+
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12), (torch.float32, 2e-6)])
+def test_persistent_kernel_matches_batched_launches(dtype, tol):
+    """Circuits of 4-9 qubits: the narrow ones run in the persistent
+    whole-circuit kernels (one forward + one adjoint launch), the wide ones
+    in batched launches; populations and every parameter gradient equal the
+    all-batched-launch run."""
+    from quantumsim_amd import persistent
+    circuits = random_circuit_batch(16, seed=21, n_range=(4, 9), layer_range=(3, 9))
+    cb = CircuitBatch(circuits, dtype=dtype, checkpoint_every=4)
+    assert cb.small and len(cb.small) < len(cb.groups)
+    assert persistent.max_qubits(dtype) >= 6
+    ref = CircuitBatch(circuits, dtype=dtype, checkpoint_every=4, persistent_kernel=False)
+    w = cb.random_weights(3)
+    _, pops, grads = cb.step(w)
+    _, pops_r, grads_r = ref.step(w)
+    for a, b in zip(pops, pops_r):
+        assert (a - b).abs().max().item() < tol
+    for a, b in zip(grads, grads_r):
+        scale = b.abs().max().item() + 1e-30
+        assert (a - b).abs().max().item() < (1e-10 if dtype == torch.float64 else 1e-4) * scale
