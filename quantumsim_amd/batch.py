@@ -29,7 +29,7 @@ import numpy as np
 import torch
 
 from . import persistent
-from .autograd import run_circuit, diagonal
+from .autograd import Gate, run_circuit, diagonal
 from .autotune import tuning
 from .bases import general
 from .workloads import rotation_ptms, damping_ptms, fused_layer_gates, layer_targets
@@ -154,11 +154,31 @@ class CircuitBatch:
     def parameters(self):
         return [t for g in self.groups for t in (g.angles, g.t1, g.t2)]
 
+    def _layer_ptms(self, g):
+        """Fused PTMs of every layer of group g, computed for all layers of a
+        parity in one batch: per parity p, a list over the layer's gates of
+        (per-layer [B, D, D] views, targets); layer l is entry l // 2 of
+        parity l % 2."""
+        n, B = g.n, len(g.members)
+        rot = rotation_ptms(g.angles)            # [B, L, n, 4, 4]
+        damp = damping_ptms(g.t1, g.t2)          # [B, n, 4, 4]
+        out = []
+        for p in (0, 1):
+            Lp = len(range(p, g.layers[0], 2))
+            if Lp == 0:
+                out.append([])
+                continue
+            rot_p = rot[:, p::2].reshape(B * Lp, n, 4, 4)
+            damp_p = damp[:, None].expand(B, Lp, n, 4, 4).reshape(B * Lp, n, 4, 4)
+            # unbind: one view per layer, whose backward is a single stack
+            out.append([(gt.ptm.reshape(B, Lp, *gt.ptm.shape[1:]).unbind(1), gt.targets)
+                        for gt in fused_layer_gates(n, p, rot_p, damp_p)])
+        return out
+
     def _group_populations(self, g):
         n, B = g.n, len(g.members)
         N = 4 ** n
-        rot = rotation_ptms(g.angles)            # [B, L, n, 4, 4]
-        damp = damping_ptms(g.t1, g.t2)          # [B, n, 4, 4]
+        ptms = self._layer_ptms(g)
         x = torch.zeros((B, N), dtype=self.dtype, device=self.device)
         x[:, 0] = 1
         finals = [None] * B
@@ -169,7 +189,7 @@ class CircuitBatch:
             end = g.layers[active - 1]
             gates = []
             for l in range(layer, end):
-                gates += fused_layer_gates(n, l, rot[:active, l], damp[:active])
+                gates += [Gate(P[l // 2][:active], t) for P, t in ptms[l % 2]]
             x = run_circuit(x[:active], gates, (4,) * n, self.k)
             layer = end
             nxt = active

@@ -15,9 +15,15 @@
 //   adjoint:  Y = dL/dS (the population gradient scattered onto those
 //             entries); per gate, last first: load the state before the gate
 //             from the trajectory, dL/dP[o][i] = sum over orbits of
-//             Y[o] X[i] (float64, fixed order: threads own (o, i) entries and
-//             slices of the orbits, slices summed in order), then Y <- P^T Y
-//             per orbit in place.
+//             Y[o] X[i] on v_mfma_f64_16x16x4 (lane (c, k) supplies entry c
+//             of orbit k: four orbits per k-step for 16 x 16 PTMs, sixteen
+//             for 4 x 4 ones packed as four 4 x 4 diagonal blocks; float64
+//             products and sums, waves summed in a fixed order), then
+//             Y <- P^T Y per orbit in place.
+//
+// LDS holds the states XOR-swizzled (circ_swz): the entries of an orbit sit
+// 4^q apart, and without the swizzle the sixteen entries the MFMA operand
+// loads of one orbit touch would share an LDS bank.
 //
 // All qubits are in the general(2) basis (Pauli dimension 4: two bits per
 // digit), qubit 0 most significant.  Gates act on 1 or 2 qubits with PTMs
@@ -26,6 +32,7 @@
 // PauliVector object there); the per-gate arithmetic is its apply_ptm
 // (quantumsim/pauli_vectors/numpy.py:50-64).
 #include "qsv_common.h"
+#include "qsv_mfma.h"
 
 #include <algorithm>
 #include <vector>
@@ -34,6 +41,11 @@ namespace qsv {
 
 constexpr int kCircBlock = 256;
 constexpr int kCircMaxQubits = 8;
+// Largest state the adjoint's register prefetch covers: 7 qubits in float32
+// (160 KB of LDS holds the adjoint's two states only up to there), 6 in
+// float64.
+template <typename T>
+constexpr uint32_t kCircMaxEntries = sizeof(T) == 4 ? (1u << 14) : (1u << 12);
 
 // Index of orbit m with zero 2-bit digits inserted at bit positions lo < hi
 // (hi < 0: one target).
@@ -54,6 +66,13 @@ __host__ __device__ inline uint32_t circ_spread(uint32_t j) {
   uint32_t r = 0;
   for (int t = 0; t < 16; ++t) r |= ((j >> t) & 1u) << (2 * t);
   return r;
+}
+
+// LDS position of state entry a: the low 6 bits XORed with a hash of the
+// higher ones (a bijection on [0, 4^n); entries below 64 stay in place).
+__host__ __device__ inline uint32_t circ_swz(uint32_t a) {
+  const uint32_t h = a >> 6;
+  return a ^ ((h ^ (h >> 6) ^ (h >> 12)) & 63u);
 }
 
 struct GateGeom {
@@ -86,13 +105,13 @@ __device__ inline void circ_orbits(T* S, const T* P, const GateGeom& G, int tid)
     const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
     T xin[D];
 #pragma unroll
-    for (int i = 0; i < D; ++i) xin[i] = S[base + circ_offset(G.k, i, G.s0, G.s1)];
+    for (int i = 0; i < D; ++i) xin[i] = S[circ_swz(base + circ_offset(G.k, i, G.s0, G.s1))];
 #pragma unroll
     for (int o = 0; o < D; ++o) {
       T acc = T(0);
 #pragma unroll
       for (int i = 0; i < D; ++i) acc += (TRANS ? P[i * D + o] : P[o * D + i]) * xin[i];
-      S[base + circ_offset(G.k, o, G.s0, G.s1)] = acc;
+      S[circ_swz(base + circ_offset(G.k, o, G.s0, G.s1))] = acc;
     }
   }
 }
@@ -110,15 +129,22 @@ __global__ void __launch_bounds__(kCircBlock)
     const qsv_circuit cd = circs[c];
     const int n = cd.n;
     const uint32_t N = 1u << (2 * n);
-    for (uint32_t i = tid; i < N; i += kCircBlock) S[i] = i == 0 ? T(1) : T(0);
+    for (uint32_t i = tid; i < N; i += kCircBlock) S[circ_swz(i)] = i == 0 ? T(1) : T(0);
+    // one gate of lookahead: its descriptor and this thread's PTM entry
+    qsv_gate nxt = cd.n_gates > 0 ? gates[cd.gate0] : qsv_gate{};
+    T pr = cd.n_gates > 0 && tid < (nxt.k == 1 ? 16 : 256) ? ptms[nxt.ptm + tid] : T(0);
     __syncthreads();
     for (int g = 0; g < cd.n_gates; ++g) {
-      const qsv_gate gd = gates[cd.gate0 + g];
+      const qsv_gate gd = nxt;
       const GateGeom G = circ_gate_geom(gd, n);
       T* tr = traj + cd.traj + int64_t(g) * N;
-      for (uint32_t i = tid; i < N; i += kCircBlock) tr[i] = S[i];
-      for (int i = tid; i < G.D * G.D; i += kCircBlock) P[i] = ptms[gd.ptm + i];
+      for (uint32_t i = tid; i < N; i += kCircBlock) tr[i] = S[circ_swz(i)];
+      if (tid < G.D * G.D) P[tid] = pr;
       __syncthreads();
+      if (g + 1 < cd.n_gates) {
+        nxt = gates[cd.gate0 + g + 1];
+        if (tid < (nxt.k == 1 ? 16 : 256)) pr = ptms[nxt.ptm + tid];
+      }
       if (G.k == 1)
         circ_orbits<T, 4, false>(S, P, G, tid);
       else
@@ -126,7 +152,7 @@ __global__ void __launch_bounds__(kCircBlock)
       __syncthreads();
     }
     for (uint32_t j = tid; j < (1u << n); j += kCircBlock)
-      pops[cd.pop + j] = double(S[circ_spread(j)]);
+      pops[cd.pop + j] = double(S[circ_swz(circ_spread(j))]);
     __syncthreads();
   }
 }
@@ -138,43 +164,94 @@ __global__ void __launch_bounds__(kCircBlock)
                    const T* __restrict__ traj, const double* __restrict__ dpops,
                    double* __restrict__ dptms, int max_qubits) {
   extern __shared__ __align__(16) unsigned char qsv_circ_smem[];
-  double* red = reinterpret_cast<double*>(qsv_circ_smem);
-  T* Y = reinterpret_cast<T*>(red + kCircBlock);
+  double* red = reinterpret_cast<double*>(qsv_circ_smem);  // [waves][64 lanes][4]
+  T* Y = reinterpret_cast<T*>(red + kCircBlock * 4);
   T* X = Y + (1u << (2 * max_qubits));
   T* P = X + (1u << (2 * max_qubits));
-  const int tid = threadIdx.x;
-  for (int c = blockIdx.x; c < n_circ; c += gridDim.x) {
-    const qsv_circuit cd = circs[c];
+  typedef typename Mfma<double>::acc Acc;
+  constexpr int W = kCircBlock / 64;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, c = lane & 15, kk = lane >> 4;
+  // the state before the next gate (backwards) is fetched into registers
+  // while the current gate computes: entries j * 256 + tid
+  constexpr int PER = kCircMaxEntries<T> / kCircBlock;
+  T xr[PER];
+  auto fetch = [&](const T* tr, uint32_t N) {
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (uint32_t(j * kCircBlock + tid) < N) xr[j] = tr[j * kCircBlock + tid];
+  };
+  for (int ci = blockIdx.x; ci < n_circ; ci += gridDim.x) {
+    const qsv_circuit cd = circs[ci];
     const int n = cd.n;
     const uint32_t N = 1u << (2 * n);
+    if (cd.n_gates > 0) fetch(traj + cd.traj + int64_t(cd.n_gates - 1) * N, N);
     for (uint32_t i = tid; i < N; i += kCircBlock) Y[i] = T(0);
     __syncthreads();
-    for (uint32_t j = tid; j < (1u << n); j += kCircBlock) Y[circ_spread(j)] = T(dpops[cd.pop + j]);
+    for (uint32_t j = tid; j < (1u << n); j += kCircBlock)
+      Y[circ_swz(circ_spread(j))] = T(dpops[cd.pop + j]);
     __syncthreads();
+    qsv_gate nxt = cd.n_gates > 0 ? gates[cd.gate0 + cd.n_gates - 1] : qsv_gate{};
+    T pr = cd.n_gates > 0 && tid < (nxt.k == 1 ? 16 : 256) ? ptms[nxt.ptm + tid] : T(0);
     for (int g = cd.n_gates - 1; g >= 0; --g) {
-      const qsv_gate gd = gates[cd.gate0 + g];
+      const qsv_gate gd = nxt;
       const GateGeom G = circ_gate_geom(gd, n);
-      const T* tr = traj + cd.traj + int64_t(g) * N;
-      for (uint32_t i = tid; i < N; i += kCircBlock) X[i] = tr[i];
-      for (int i = tid; i < G.D * G.D; i += kCircBlock) P[i] = ptms[gd.ptm + i];
+#pragma unroll
+      for (int j = 0; j < PER; ++j)
+        if (uint32_t(j * kCircBlock + tid) < N) X[circ_swz(j * kCircBlock + tid)] = xr[j];
+      if (tid < G.D * G.D) P[tid] = pr;
       __syncthreads();
-      // dL/dP: thread = (entry e, orbit slice)
-      const int E = G.D * G.D, slices = kCircBlock / E;
-      {
-        const int e = tid % E, sl = tid / E;
-        const uint32_t oo = circ_offset(G.k, e / G.D, G.s0, G.s1);
-        const uint32_t io = circ_offset(G.k, e % G.D, G.s0, G.s1);
-        double acc = 0.0;
-        for (uint32_t m = sl; m < G.n_orbits; m += slices) {
-          const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
-          acc += double(Y[base + oo]) * double(X[base + io]);
-        }
-        red[tid] = acc;
+      if (g > 0) {
+        fetch(traj + cd.traj + int64_t(g - 1) * N, N);
+        nxt = gates[cd.gate0 + g - 1];
+        if (tid < (nxt.k == 1 ? 16 : 256)) pr = ptms[nxt.ptm + tid];
       }
+      // dL/dP on MFMA: A[row][k] = Y entry, B[k][col] = X entry, k = orbit
+      Acc acc = Acc(0);
+      if (G.k == 2) {
+        const uint32_t oc = circ_offset(2, c, G.s0, G.s1);
+        for (uint32_t m0 = 4 * w; m0 < G.n_orbits; m0 += 4 * W) {
+          const uint32_t m = m0 + kk;
+          double a = 0.0, b = 0.0;
+          if (m < G.n_orbits) {
+            const uint32_t e = circ_swz(circ_orbit_base(m, G.lo, G.hi) + oc);
+            a = double(Y[e]);
+            b = double(X[e]);
+          }
+          acc = Mfma<double>::op(a, b, acc);
+        }
+      } else {
+        // sixteen orbits per k-step: lane column c = 4 u + (entry), orbit
+        // m0 + 4 kk + u; dL/dP = sum of the four 4 x 4 diagonal blocks
+        const uint32_t oc = circ_offset(1, c & 3, G.s0, 0);
+        for (uint32_t m0 = 16 * w; m0 < G.n_orbits; m0 += 16 * W) {
+          const uint32_t m = m0 + 4 * kk + (c >> 2);
+          double a = 0.0, b = 0.0;
+          if (m < G.n_orbits) {
+            const uint32_t e = circ_swz(circ_orbit_base(m, G.lo, G.hi) + oc);
+            a = double(Y[e]);
+            b = double(X[e]);
+          }
+          acc = Mfma<double>::op(a, b, acc);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[(w * 64 + lane) * 4 + r] = acc[r];
       __syncthreads();
-      if (tid < E) {
+      // entry (o, i): float64 accumulator layout, row = lane / 16 + 4 r,
+      // column = lane % 16
+      if (tid < G.D * G.D) {
         double sum = 0.0;
-        for (int sl = 0; sl < slices; ++sl) sum += red[sl * E + tid];
+        if (G.k == 2) {
+          const int o = tid >> 4, i = tid & 15;
+          const int src = (i + 16 * (o & 3)) * 4 + (o >> 2);
+          for (int ww = 0; ww < W; ++ww) sum += red[ww * 256 + src];
+        } else {
+          const int o = tid >> 2, i = tid & 3;
+          for (int u = 0; u < 4; ++u) {
+            const int src = (4 * u + i + 16 * o) * 4 + u;  // row 4 u + o, column 4 u + i
+            for (int ww = 0; ww < W; ++ww) sum += red[ww * 256 + src];
+          }
+        }
         dptms[gd.ptm + tid] = sum;
       }
       // Y <- P^T Y, orbit by orbit in place
@@ -190,7 +267,7 @@ __global__ void __launch_bounds__(kCircBlock)
 static size_t circ_lds_bytes(int elem_bytes, int max_qubits, bool adjoint) {
   const size_t N = size_t(1) << (2 * max_qubits);
   if (!adjoint) return (N + 256) * elem_bytes;
-  return kCircBlock * 8 + (2 * N + 256) * elem_bytes;
+  return kCircBlock * 4 * 8 + (2 * N + 256) * elem_bytes;
 }
 
 static int lds_limit() {
@@ -227,9 +304,11 @@ using namespace qsv;
 extern "C" int qsv_circuits_max_qubits(int dtype) {
   const int es = dtype_size(dtype);
   QSV_CHECK_ARG(es > 0, "unknown dtype %d", dtype);
+  const uint32_t max_entries = es == 4 ? kCircMaxEntries<float> : kCircMaxEntries<double>;
   int best = 0;
   for (int n = 1; n <= kCircMaxQubits; ++n)
-    if (circ_lds_bytes(es, n, true) <= size_t(lds_limit())) best = n;
+    if (circ_lds_bytes(es, n, true) <= size_t(lds_limit()) && (1u << (2 * n)) <= max_entries)
+      best = n;
   return best;
 }
 
@@ -329,6 +408,14 @@ extern "C" int qsv_emulate_circuits(int n_circuits, const qsv_circuit* circs,
     QSV_CHECK_ARG(cd.n_gates >= 0 && cd.gate0 >= 0 && cd.gate0 + cd.n_gates <= n_gates_total,
                   "circuit %d: gate range out of the table", c);
     const uint32_t N = 1u << (2 * cd.n);
+    {
+      std::vector<char> seen(N, 0);
+      for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t p = circ_swz(i);
+        QSV_CHECK_ARG(p < N && !seen[p], "LDS swizzle not a bijection for %d qubits", cd.n);
+        seen[p] = 1;
+      }
+    }
     QSV_CHECK_ARG(cd.traj >= 0 && cd.traj + int64_t(cd.n_gates) * N <= traj_elems,
                   "circuit %d: trajectory out of range", c);
     QSV_CHECK_ARG(cd.pop >= 0 && cd.pop + (int64_t(1) << cd.n) <= pop_elems,
@@ -384,21 +471,52 @@ extern "C" int qsv_emulate_circuits(int n_circuits, const qsv_circuit* circs,
       const qsv_gate gd = gates[cd.gate0 + g];
       const GateGeom G = circ_gate_geom(gd, cd.n);
       for (uint32_t i = 0; i < N; ++i) X[i] = traj[cd.traj + int64_t(g) * N + i];
-      const int E = G.D * G.D, slices = kCircBlock / E;
-      std::vector<double> red(kCircBlock, 0.0);
-      for (int tid = 0; tid < kCircBlock; ++tid) {
-        const int e = tid % E, sl = tid / E;
-        const uint32_t oo = circ_offset(G.k, e / G.D, G.s0, G.s1);
-        const uint32_t io = circ_offset(G.k, e % G.D, G.s0, G.s1);
-        for (uint32_t m = sl; m < G.n_orbits; m += slices) {
-          const uint32_t base = circ_orbit_base(m, G.lo, G.hi);
-          red[tid] += Y[base + oo] * X[base + io];
+      // MFMA replay: waves, k-steps and lanes as in k_circuits_adj
+      constexpr int W = kCircBlock / 64;
+      std::vector<double> red(size_t(W) * 64 * 4, 0.0);
+      std::vector<int> used(G.n_orbits, 0);
+      const uint32_t step = G.k == 2 ? 4 : 16;
+      for (int w = 0; w < W; ++w)
+        for (uint32_t m0 = step * w; m0 < G.n_orbits; m0 += step * W) {
+          double A[64], B[64];
+          for (int lane = 0; lane < 64; ++lane) {
+            const int c = lane & 15, kk = lane >> 4;
+            const uint32_t m = G.k == 2 ? m0 + kk : m0 + 4 * kk + (c >> 2);
+            const uint32_t oc = G.k == 2 ? circ_offset(2, c, G.s0, G.s1)
+                                         : circ_offset(1, c & 3, G.s0, 0);
+            A[lane] = B[lane] = 0.0;
+            if (m < G.n_orbits) {
+              const uint32_t e = circ_orbit_base(m, G.lo, G.hi) + oc;
+              QSV_CHECK_ARG(e < N, "adjoint: LDS index out of range");
+              A[lane] = Y[e];
+              B[lane] = X[e];
+              if (G.k == 2 ? c == 0 : (c & 3) == 0) ++used[m];
+            }
+          }
+          for (int lane = 0; lane < 64; ++lane)
+            for (int r = 0; r < 4; ++r) {
+              const int row = (lane >> 4) + 4 * r, col = lane & 15;
+              double d = 0.0;
+              for (int q = 0; q < 4; ++q) d += A[row + 16 * q] * B[col + 16 * q];
+              red[(w * 64 + lane) * 4 + r] += d;
+            }
         }
-      }
-      for (int e = 0; e < E; ++e) {
+      for (uint32_t m = 0; m < G.n_orbits; ++m)
+        QSV_CHECK_ARG(used[m] == 1, "adjoint: orbit %u used %d times", m, used[m]);
+      for (int tid = 0; tid < G.D * G.D; ++tid) {
         double sum = 0.0;
-        for (int sl = 0; sl < slices; ++sl) sum += red[sl * E + e];
-        dptms[gd.ptm + e] = sum;
+        if (G.k == 2) {
+          const int o = tid >> 4, i = tid & 15;
+          const int src = (i + 16 * (o & 3)) * 4 + (o >> 2);
+          for (int ww = 0; ww < W; ++ww) sum += red[ww * 256 + src];
+        } else {
+          const int o = tid >> 2, i = tid & 3;
+          for (int u = 0; u < 4; ++u) {
+            const int src = (4 * u + i + 16 * o) * 4 + u;
+            for (int ww = 0; ww < W; ++ww) sum += red[ww * 256 + src];
+          }
+        }
+        dptms[gd.ptm + tid] = sum;
       }
       for (uint32_t m = 0; m < G.n_orbits; ++m) {
         const uint32_t base = circ_orbit_base(m, G.lo, G.hi);

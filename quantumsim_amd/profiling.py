@@ -9,13 +9,17 @@ plain copy, then runs the kernels under study.
   apply2q : the 13 two-qubit PTM applications of one brickwork layer pair
   apply1q : one-qubit PTM applications on qubits 0, n/2, n-1
   fused   : dense 3- and 4-qubit PTMs (MFMA kernel) on middle and inner qubits
-  batched : one forward + adjoint step of the 512-circuit batch
+  batched : the 512-circuit batch tuned and captured, then a marker kernel
+            (k_scale of 4,099 elements) and graph replays of forward + adjoint
 """
 import argparse
 import os
 import sys
 
 import numpy as np
+
+
+MARKER_ELEMS = 4099
 
 
 def main(argv=None):
@@ -44,10 +48,20 @@ def main(argv=None):
     if a.what == 'batched':
         from quantumsim_amd.batch import CircuitBatch
         from quantumsim_amd.workloads import random_circuit_batch
-        cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32)
+        cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32,
+                          checkpoint_every=16)
         w = cb.random_weights(7)
+        cb.capture(w, autotune=True)       # tuning + warm-up + capture
+        torch.cuda.synchronize()
+        # marker: one k_scale launch (not used by the batch); the measured
+        # replays follow it
+        from quantumsim_amd import _native as nat
+        m = torch.ones(MARKER_ELEMS, dtype=torch.float64, device='cuda')
+        nat.check(nat.lib().qsv_scale(nat.QSV_F64, m.data_ptr(), m.numel(), 1.0,
+                                      torch.cuda.current_stream().cuda_stream), 'scale')
+        torch.cuda.synchronize()
         for _ in range(a.reps):
-            cb.step(w)
+            cb.replay()
         torch.cuda.synchronize()
         print('done', flush=True)
         return 0

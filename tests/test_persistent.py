@@ -78,3 +78,24 @@ def test_table_rejects_bad_input():
     t = CircuitTable([(2, [((0,), 0), ((1,), 8)])], 32)  # two gates sharing PTM entries
     with pytest.raises(ValueError):
         emulate(t, np.zeros(32))
+
+
+def test_wide_circuits_and_swizzle():
+    """6- and 7-qubit circuits (LDS swizzle over the full 4^7 range, every
+    target position) against the oracle, plus the adjoint at 5 qubits."""
+    rng = np.random.default_rng(11)
+    circuits, pool = _random_program(rng, [6, 7], 6)
+    table = CircuitTable(circuits, pool.size)
+    pops, _ = emulate(table, pool)
+    assert np.allclose(pops, _oracle_pops(circuits, pool), rtol=1e-12, atol=1e-12)
+    circuits, pool = _random_program(rng, [5], 4)
+    table = CircuitTable(circuits, pool.size)
+    w = rng.standard_normal(table.pop_elems)
+    _, dpool = emulate(table, pool, w)
+    h = 1e-4
+    for idx in rng.choice(pool.size, 8, replace=False):
+        p1, p2 = pool.copy(), pool.copy()
+        p1[idx] += h
+        p2[idx] -= h
+        fd = (w @ _oracle_pops(circuits, p1) - w @ _oracle_pops(circuits, p2)) / (2 * h)
+        assert abs(dpool[idx] - fd) < 1e-9 * max(1.0, abs(fd))
