@@ -9,6 +9,7 @@ plain copy, then runs the kernels under study.
   apply2q : the 13 two-qubit PTM applications of one brickwork layer pair
   apply1q : one-qubit PTM applications on qubits 0, n/2, n-1
   fused   : dense 3- and 4-qubit PTMs (MFMA kernel) on middle and inner qubits
+  diagonal: the float64 population readout (2^n-entry gather) of the state
   batched : the 512-circuit batch tuned and captured, then a marker kernel
             (k_scale of 4,099 elements) and graph replays of forward + adjoint
 """
@@ -24,7 +25,7 @@ MARKER_ELEMS = 4099
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument('what', choices=['apply2q', 'apply1q', 'fused', 'batched'])
+    ap.add_argument('what', choices=['apply2q', 'apply1q', 'fused', 'batched', 'diagonal'])
     ap.add_argument('--qubits', type=int, default=14)
     ap.add_argument('--dtype', default='float64')
     ap.add_argument('--reps', type=int, default=3)
@@ -67,6 +68,12 @@ def main(argv=None):
         return 0
     v = qa.PauliVectorHIP([bases.general(2)] * n, force=True, dtype=dt)
     v.to_torch().copy_(torch.randn((4,) * n, dtype=dt, device='cuda'))
+    if a.what == 'diagonal':
+        for _ in range(a.reps):
+            v.diagonal(get_data=False)
+        torch.cuda.synchronize()
+        print('done: %d diagonal readouts' % a.reps, flush=True)
+        return 0
     circuit, _ = noisy_layers(n, 2, seed=1, pairing='brickwork')
     if a.what == 'apply2q':
         gates = [(p, q) for p, q in circuit if len(q) == 2]
