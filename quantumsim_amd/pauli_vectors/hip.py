@@ -457,6 +457,36 @@ class PauliVectorHIP(PauliVectorBase):
                 "will fail. Have you projected DM on a state with zero weight?")
         return tr
 
+    @classmethod
+    def from_dm(cls, dm, bases, *, force=False, dtype=None, device=None):
+        """From a density matrix, converted on the device
+        (quantumsim/pauli_vectors/pauli_vector.py:48-53 with
+        quantumsim/algebra/algebra.py:44-53; the qudits may have different
+        Hilbert dimensions)."""
+        from ..algebra import device as dalg
+        if not hasattr(bases, '__iter__'):
+            n_qubits = len(dm) // bases.dim_hilbert      # the reference's own rule
+            bases = [bases] * n_qubits
+        bases = list(bases)
+        dev = _resolve_device(device)
+        return cls(bases, dalg.dm_to_pv(dm, bases, device=dev), force=force, dtype=dtype,
+                   device=dev)
+
+    def to_dm(self):
+        """Density matrix (numpy complex), converted on the device
+        (quantumsim/pauli_vectors/pauli_vector.py:55-56, algebra.py:56-62)."""
+        from ..algebra import device as dalg
+        self._check_bases_match()
+        return dalg.pv_to_dm(self.to_torch(), self.bases, device=self._device).cpu().numpy()
+
+    def exp_value(self, operator):
+        """tr(O rho) on the device, as quantumsim/states/state.py:43-83
+        computes it for a State: ``operator`` a string of I, X, Y, Z or an
+        operator matrix; returns a complex number."""
+        from ..algebra import device as dalg
+        self._check_bases_match()
+        return dalg.exp_value(self.to_torch(), self.bases, operator, device=self._device)
+
     def copy(self):
         """Deep copy on the same device (the bases list is new, the basis
         objects are shared, as in numpy.py:119-120)."""
