@@ -124,6 +124,20 @@ int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
                       const int32_t* dims_out_t, const void* ptm_dev,
                       void* workspace, int64_t workspace_bytes, void* stream);
 
+/* qsv_apply_ptm_dev on strided tensors (strides as in qsv_apply_ptm_strided).
+ * Dense 64 x 64 / 256 x 256 PTMs are packed into the MFMA kernels' fragment
+ * order in the workspace by a device kernel on `stream`, so a PTM built on
+ * the device never travels to the host.  This is the backend call of the
+ * device-side operation pipeline: the PTM of quantumsim's
+ * PTMOperation.__call__ (quantumsim/operations/operation.py:529-551),
+ * built and basis-converted with torch, applied as
+ * PauliVectorNumpy.apply_ptm does (quantumsim/pauli_vectors/numpy.py:50-64). */
+int qsv_apply_ptm_dev_strided(int dtype, const void* in, void* out, int rank,
+                              const int64_t* dims, const int64_t* in_strides,
+                              const int64_t* out_strides, int k, const int32_t* targets,
+                              const int32_t* dims_out_t, const void* ptm_dev,
+                              void* workspace, int64_t workspace_bytes, void* stream);
+
 /* Batched form of qsv_apply_ptm_dev for PTMs up to 16 x 16: `batch` states
  * of identical shape, stored back to back (in: batch x N_in, out: batch x
  * N_out), each contracted with its own device PTM

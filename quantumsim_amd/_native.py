@@ -57,6 +57,9 @@ _SIGNATURES = {
     'qsv_apply_ptm_dev': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, _c_i64p,
                                          ctypes.c_int, _c_i32p, _c_i32p, _vp, _vp,
                                          ctypes.c_int64, _vp]),
+    'qsv_apply_ptm_dev_strided': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, _c_i64p,
+                                                 _c_i64p, _c_i64p, ctypes.c_int, _c_i32p,
+                                                 _c_i32p, _vp, _vp, ctypes.c_int64, _vp]),
     'qsv_apply_ptm_batched': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                              _c_i64p, ctypes.c_int, _c_i32p, _c_i32p, _vp,
                                              ctypes.c_int64, _vp]),

@@ -1044,12 +1044,14 @@ extern "C" int qsv_apply_ptm_strided(int dtype, const void* in, void* out, int r
                           : run_host_table<float>(p, ptm_f64, workspace, in, out, s);
 }
 
-extern "C" int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
-                                 const int64_t* dims, int k, const int32_t* targets,
-                                 const int32_t* dims_out_t, const void* ptm_dev,
-                                 void* workspace, int64_t workspace_bytes, void* stream) {
+extern "C" int qsv_apply_ptm_dev_strided(int dtype, const void* in, void* out, int rank,
+                                         const int64_t* dims, const int64_t* in_strides,
+                                         const int64_t* out_strides, int k,
+                                         const int32_t* targets, const int32_t* dims_out_t,
+                                         const void* ptm_dev, void* workspace,
+                                         int64_t workspace_bytes, void* stream) {
   Plan p;
-  const int rc = make_plan(dtype, rank, dims, nullptr, nullptr, k, targets, dims_out_t, p);
+  const int rc = make_plan(dtype, rank, dims, in_strides, out_strides, k, targets, dims_out_t, p);
   if (rc) return rc;
   QSV_CHECK_ARG(in && out && ptm_dev, "null pointer argument");
   QSV_CHECK_ARG(in != out, "in and out must not alias");
@@ -1062,12 +1064,31 @@ extern "C" int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
     return small ? run_ptr<float, 4>(p, ptm_dev, in, out, s)
                  : run_ptr<float, 16>(p, ptm_dev, in, out, s);
   }
+  FusedPlan fp;
+  if (make_fused_plan(p, dtype_size(dtype), in, out, fp)) {
+    // dense 64 x 64 / 256 x 256: the PTM is packed into MFMA fragment order
+    // in the workspace by a kernel on the same stream (no host copy)
+    const int64_t need = int64_t(fp.D) * fp.D * dtype_size(dtype);
+    QSV_CHECK_ARG(workspace && workspace_bytes >= need,
+                  "a %d x %d PTM needs a workspace of %lld bytes", fp.D, fp.D, (long long)need);
+    const int rp = launch_fused_pack(dtype, fp, ptm_dev, workspace, s);
+    if (rp) return rp;
+    return launch_fused(dtype, fp, p.g, p.nt, in, out, workspace, s);
+  }
   const int64_t need = table_bytes(dtype, p.din, p.dout, false);
   QSV_CHECK_ARG(workspace && workspace_bytes >= need,
                 "offset tables of a %d x %d PTM need a workspace of %lld bytes", p.dout,
                 p.din, (long long)need);
   return dtype == QSV_F64 ? run_dev_table<double>(p, ptm_dev, workspace, in, out, s)
                           : run_dev_table<float>(p, ptm_dev, workspace, in, out, s);
+}
+
+extern "C" int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
+                                 const int64_t* dims, int k, const int32_t* targets,
+                                 const int32_t* dims_out_t, const void* ptm_dev,
+                                 void* workspace, int64_t workspace_bytes, void* stream) {
+  return qsv_apply_ptm_dev_strided(dtype, in, out, rank, dims, nullptr, nullptr, k, targets,
+                                   dims_out_t, ptm_dev, workspace, workspace_bytes, stream);
 }
 
 // ------------------------------------------------------ host emulation

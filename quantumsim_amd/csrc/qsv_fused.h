@@ -69,20 +69,26 @@ __host__ __device__ inline int64_t fused_frag_slot(int D, int s, int mb, int lan
   return (int64_t(s) * (D / 16) + mb) * 64 + lane;
 }
 
+// Row-major PTM index (o * D + k) whose value sits at fragment slot `slot`
+// (the inverse of fused_frag_slot composed with the row/column maps).
+__host__ __device__ inline int64_t fused_frag_source(int D, int elem_bytes, int vi, int vo,
+                                                     int64_t slot) {
+  const int MB = D / 16;
+  const int lane = int(slot & 63);
+  const int64_t sm = slot >> 6;
+  const int mb = int(sm % MB);
+  const int s = int(sm / MB);
+  const int m = lane & 15, jk = lane >> 4;
+  int j, r;
+  mfma_row_owner(elem_bytes, m, j, r);
+  return int64_t(fused_o(vo, 4 * mb + r, j)) * D + fused_k(vi, s, jk);
+}
+
 // Fills pf[D * D] (fragment order) from the row-major float64 PTM.
 template <typename T>
 inline void fused_fragments(int D, int elem_bytes, int vi, int vo, const double* ptm, T* pf) {
-  const int MB = D / 16, KS = D / 4;
-  for (int s = 0; s < KS; ++s)
-    for (int mb = 0; mb < MB; ++mb)
-      for (int lane = 0; lane < 64; ++lane) {
-        const int m = lane & 15, jk = lane >> 4;
-        int j, r;
-        mfma_row_owner(elem_bytes, m, j, r);
-        const int o = fused_o(vo, 4 * mb + r, j);
-        const int k = fused_k(vi, s, jk);
-        pf[fused_frag_slot(D, s, mb, lane)] = T(ptm[int64_t(o) * D + k]);
-      }
+  for (int64_t slot = 0; slot < int64_t(D) * D; ++slot)
+    pf[slot] = T(ptm[fused_frag_source(D, elem_bytes, vi, vo, slot)]);
 }
 
 // k-steps per LDS chunk of the 256 x 256 kernel: a chunk covers 16 (float64)
@@ -93,6 +99,12 @@ constexpr int fused_chunk_steps(int elem_bytes) { return elem_bytes == 8 ? 4 : 8
 
 int launch_fused(int dtype, const FusedPlan& fp, const Geom& g, bool nt, const void* in,
                  void* out, const void* ptm_frag, hipStream_t s);
+
+// Packs a device PTM (row-major D x D, state's element type) into fragment
+// order at pf (D * D elements) on stream s; the device form of
+// fused_fragments.
+int launch_fused_pack(int dtype, const FusedPlan& fp, const void* ptm_dev, void* pf,
+                      hipStream_t s);
 
 // Column blocks per wave tile: the 64 x 64 kernel gives each wave tiles of
 // 16 * nb orbits, the 256 x 256 kernel each block tiles of 4 x 16 orbits

@@ -315,6 +315,33 @@ static int dispatch(const FusedPlan& fp, const Geom& g, const void* in, void* ou
   }
 }
 
+// Device form of fused_fragments: pf[slot] = ptm[o * D + k] for the slot's
+// (k-step, row block, lane), both arrays in the state's element type.  One
+// thread per slot; D * D slots.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_fused_pack(const T* __restrict__ ptm, T* __restrict__ pf, int D, int vi, int vo) {
+  const int64_t slot = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (slot >= int64_t(D) * D) return;
+  pf[slot] = ptm[fused_frag_source(D, int(sizeof(T)), vi, vo, slot)];
+}
+
+int launch_fused_pack(int dtype, const FusedPlan& fp, const void* ptm_dev, void* pf,
+                      hipStream_t s) {
+  const int64_t n = int64_t(fp.D) * fp.D;
+  const dim3 grid(uint32_t((n + kBlock - 1) / kBlock));
+  if (dtype == QSV_F64)
+    hipLaunchKernelGGL(k_fused_pack<double>, grid, dim3(kBlock), 0, s,
+                       static_cast<const double*>(ptm_dev), static_cast<double*>(pf), fp.D,
+                       fp.vi, fp.vo);
+  else
+    hipLaunchKernelGGL(k_fused_pack<float>, grid, dim3(kBlock), 0, s,
+                       static_cast<const float*>(ptm_dev), static_cast<float*>(pf), fp.D, fp.vi,
+                       fp.vo);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
 int launch_fused(int dtype, const FusedPlan& fp, const Geom& g, bool nt, const void* in,
                  void* out, const void* pf, hipStream_t s) {
   if (fp.D != 64 && fp.D != 256) {

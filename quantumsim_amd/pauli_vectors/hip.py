@@ -244,14 +244,19 @@ class PauliVectorHIP(PauliVectorBase):
         """Contract ``ptm`` (shape (out..., in...)) with the listed qudits
         (quantumsim/pauli_vectors/numpy.py:50-64).  ``bases`` is left alone;
         the caller updates it (quantumsim/operations/operation.py:549-551)."""
-        if isinstance(ptm, torch.Tensor):
+        # a PTM already on this vector's device stays there (no host copy,
+        # no synchronisation): the device-side operation pipeline
+        on_device = (isinstance(ptm, torch.Tensor) and ptm.device.type == 'cuda'
+                     and ptm.device == self._device)
+        if isinstance(ptm, torch.Tensor) and not on_device:
             ptm = ptm.detach().cpu().numpy()
-        ptm = np.asarray(ptm)
+        if not on_device:
+            ptm = np.asarray(ptm)
         k = len(qubits)
         if ptm.ndim != 2 * k:
             raise ValueError('{}-qubit PTM must have {} dimensions, got {}'
                              .format(k, 2 * k, ptm.ndim))
-        if np.iscomplexobj(ptm):
+        if (ptm.is_complex() if on_device else np.iscomplexobj(ptm)):
             raise ValueError('PTM must be real, got dtype {}'.format(ptm.dtype))
         if k == 0:
             raise ValueError('apply_ptm needs at least one qubit')
@@ -269,7 +274,10 @@ class PauliVectorHIP(PauliVectorBase):
             if self._shape[q] != d:
                 raise ValueError('PTM input dimension {} does not match dimension {} of '
                                  'qubit {}'.format(d, self._shape[q], q))
-        ptm = np.ascontiguousarray(ptm, dtype=np.float64)
+        if on_device:
+            ptm = ptm.detach().to(self._dtype).contiguous()
+        else:
+            ptm = np.ascontiguousarray(ptm, dtype=np.float64)
         new_shape = list(self._shape)
         for q, d in zip(qubits, dims_out_t):
             new_shape[q] = d
@@ -281,11 +289,18 @@ class PauliVectorHIP(PauliVectorBase):
                                                    nat.i32(dims_out_t))
         nat.check(ws_bytes, 'apply_ptm')
         ws = _Scratch.get(self._device, 'ptm', ws_bytes).data_ptr() if ws_bytes else None
-        rc = L.qsv_apply_ptm_strided(
-            code, self._store.data_ptr(), out.data_ptr(), len(self._shape),
-            nat.i64(self._shape), nat.i64(self._pstr), nat.i64(new_str), k, nat.i32(qubits),
-            nat.i32(dims_out_t), ptm.ctypes.data_as(nat._c_f64p), ws, ws_bytes,
-            _stream(self._device))
+        if on_device:
+            rc = L.qsv_apply_ptm_dev_strided(
+                code, self._store.data_ptr(), out.data_ptr(), len(self._shape),
+                nat.i64(self._shape), nat.i64(self._pstr), nat.i64(new_str), k,
+                nat.i32(qubits), nat.i32(dims_out_t), ptm.data_ptr(), ws, ws_bytes,
+                _stream(self._device))
+        else:
+            rc = L.qsv_apply_ptm_strided(
+                code, self._store.data_ptr(), out.data_ptr(), len(self._shape),
+                nat.i64(self._shape), nat.i64(self._pstr), nat.i64(new_str), k,
+                nat.i32(qubits), nat.i32(dims_out_t), ptm.ctypes.data_as(nat._c_f64p), ws,
+                ws_bytes, _stream(self._device))
         if rc != 0:
             self._spare = out
             nat.check(rc, 'apply_ptm')
