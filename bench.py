@@ -152,12 +152,21 @@ def bench_batched(args):
     for i, t in zip(sample, per):
         mean_by_n.setdefault(circuits[i]['n'], []).append(t)
     plain_sec = sum(len(by_n[n]) * statistics.mean(v) for n, v in mean_by_n.items())
+    # probability error of the float32 step against the same batch in float64
+    cb.replay()
+    torch.cuda.synchronize()
+    cb64 = CircuitBatch(circuits, dtype=torch.float64, checkpoint_every=16)
+    with torch.no_grad():
+        p64 = cb64.populations()
+    prob_err = max((a - b).abs().max().item() for a, b in zip(cb.g_pops, p64))
+    del cb64, p64
     res = {'circuits': len(circuits), 'fwd_adjoint_s': round(sec, 4),
            'circuits_per_s': round(len(circuits) / sec, 1),
            'plain_circuits_per_s': round(len(circuits) / plain_sec, 2),
            'speedup_vs_plain': round(plain_sec / sec, 2),
            'plain_sample': len(sample), 'dtype': 'f32 state, f64 readout',
            'checkpoint_every': 16, 'setup_s': round(t_setup, 1),
+           'max_prob_err_vs_f64': float('%.3e' % prob_err),
            'autotune': tuner.summary()}
     log('batched: %s' % res)
     del cb

@@ -88,7 +88,9 @@ class Autotuner:
         e = self.entries.get(key)
         if e is None and self.active:
             e = self.tune_apply(dtype, batch, shape, targets, dims_out)
-        return None if e is None else nat.Tile(e['kernel'], e['tile_orbits'], 0, 0)
+        if e is None or e['kernel'] is None:
+            return None
+        return nat.Tile(e['kernel'], e['tile_orbits'], 0, 0)
 
     def grad_tile(self, dtype, batch, shape, targets, dims_out):
         if not self.entries and not self.active:
@@ -147,8 +149,14 @@ class Autotuner:
             t = self._time(lambda: run(tile))
             if best is None or t < best[0]:
                 best = (t, kern, orbits)
-        e = {'kernel': best[1], 'tile_orbits': best[2], 'us': round(best[0], 2),
-             'default_us': round(t_default, 2)}
+        if best is None:
+            # no launch shape applies (dense 64 x 64 PTMs run on the MFMA
+            # kernel, which has none): the planner's choice
+            e = {'kernel': None, 'tile_orbits': 0, 'us': round(t_default, 2),
+                 'default_us': round(t_default, 2)}
+        else:
+            e = {'kernel': best[1], 'tile_orbits': best[2], 'us': round(best[0], 2),
+                 'default_us': round(t_default, 2)}
         self._store(self._key('apply', dtype, batch, shape, targets, dims_out), e)
         return e
 
@@ -182,6 +190,8 @@ class Autotuner:
         best = None
         for nb in GRAD_CANDIDATES:
             tile = nat.Tile(0, 0, nb, 0)
+            if run(tile) != 0:          # above the kernel's block limit (wide PTMs)
+                continue
             t = self._time(lambda: nat.check(run(tile), 'ptm_grad'))
             if best is None or t < best[0]:
                 best = (t, nb)

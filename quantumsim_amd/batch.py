@@ -11,10 +11,13 @@ carries dL/dP from the adjoint kernel to the angles and T1/T2.
 Wider circuits (whose states, 256 KB to 16 MB each, do not fit in LDS and
 are streamed at HBM bandwidth anyway) are grouped by qubit count; inside a
 group the states form one [B, 4^n] tensor ordered by decreasing depth, so
-the circuits still running at any layer are a prefix of the batch.  Every
-fused gate of a layer is one qsv_apply_ptm_batched launch over that prefix;
-circuits that have finished are retired by narrowing the prefix, without
-copies.  Forward, the float64 population readout and the adjoint
+the circuits still running at any layer are a prefix of the batch.  Layers
+are taken two at a time and their gates composed (torch, differentiable)
+into dense blocks of up to three qubits, each one qsv_apply_ptm_batched
+launch over that prefix (64 x 64 PTMs on the MFMA kernel): about half the
+passes over the states of one launch per gate.  A circuit that ends inside
+a two-layer window gets identity PTMs for its missing layer; circuits that
+have finished are retired by narrowing the prefix, without copies.  Forward, the float64 population readout and the adjoint
 (checkpointed recompute + qsv_ptm_grad) are all device work with no host
 synchronisation, so the whole step can be captured once in a HIP graph and
 replayed (``CircuitBatch.capture``).
@@ -32,6 +35,7 @@ from . import persistent
 from .autograd import Gate, run_circuit, diagonal
 from .autotune import tuning
 from .bases import general
+from .fusion import fuse_batched
 from .workloads import rotation_ptms, damping_ptms, fused_layer_gates, layer_targets
 
 __all__ = ['CircuitBatch', 'plain_forward_backward']
@@ -64,10 +68,16 @@ class CircuitBatch:
     gradients with respect to every parameter."""
 
     def __init__(self, circuits, dtype=torch.float32, device=None, checkpoint_every=16,
-                 requires_grad=True, persistent_kernel=True):
+                 requires_grad=True, persistent_kernel=True, fuse_qudits=3):
         self.device = device or torch.device('cuda', torch.cuda.current_device())
         self.dtype = dtype
         self.k = checkpoint_every
+        # circuits in batched launches: layers taken two at a time and fused
+        # into dense blocks of up to fuse_qudits qubits (3: 64 x 64 PTMs on
+        # the MFMA kernels; 2 or less: the layer's own gates, one at a time)
+        self.fuse_qudits = int(fuse_qudits or 0)
+        self.window = 2 if self.fuse_qudits > 2 else 1
+        self._consts = {}
         self.circuits = circuits
         # one stream (default priority) for parameters, eager steps and the
         # captured graph, so autograd never crosses streams
@@ -175,6 +185,46 @@ class CircuitBatch:
                         for gt in fused_layer_gates(n, p, rot_p, damp_p)])
         return out
 
+    def _cached(self, key, make):
+        """Constant device tensors (masks, identities) made once, in an eager
+        step, so a captured graph never copies from the host."""
+        t = self._consts.get(key)
+        if t is None:
+            t = make()
+            self._consts[key] = t
+        return t
+
+    def _window_gates(self, g, ptms, l, active):
+        """Gates of layers [l, l + window) for the `active` deepest circuits
+        of group g: a circuit that ends inside the window gets identity PTMs
+        for its missing layers (its state is final after the window), and
+        with fusion the window's gates are composed into dense blocks of up
+        to ``fuse_qudits`` qubits (workloads.fused_layer_gates -> at most
+        n/2 + 1 two-qubit PTMs per layer -> about n/3 three-qubit blocks per
+        two layers)."""
+        n = g.n
+        gates = []
+        for j in range(self.window):
+            lj = l + j
+            if lj >= g.layers[0]:
+                break
+            keep = [g.layers[b] > lj for b in range(active)]
+            mask = None
+            if not all(keep):
+                mask = self._cached(('mask', id(g), lj, active), lambda: torch.tensor(
+                    keep, dtype=torch.bool, device=self.device)[:, None, None])
+            for P, t in ptms[lj % 2]:
+                pt = P[lj // 2][:active]
+                if mask is not None:
+                    D = pt.shape[-1]
+                    eye = self._cached(('eye', D, pt.dtype), lambda: torch.eye(
+                        D, dtype=pt.dtype, device=self.device))
+                    pt = torch.where(mask, pt, eye)
+                gates.append(Gate(pt, t))
+        if self.fuse_qudits > 2 and len(gates) > 1:
+            gates = [Gate(m, q) for m, q in fuse_batched(gates, [4] * n, self.fuse_qudits)]
+        return gates
+
     def _group_populations(self, g):
         n, B = g.n, len(g.members)
         N = 4 ** n
@@ -182,19 +232,18 @@ class CircuitBatch:
         x = torch.zeros((B, N), dtype=self.dtype, device=self.device)
         x[:, 0] = 1
         finals = [None] * B
+        deeper = lambda layer: sum(1 for d in g.layers if d > layer)
         layer = 0
         active = B
         while active > 0:
-            # layers while the active prefix is unchanged
-            end = g.layers[active - 1]
+            # windows while the active prefix (circuits deeper than the
+            # window's first layer) is unchanged: one checkpointed segment
             gates = []
-            for l in range(layer, end):
-                gates += [Gate(P[l // 2][:active], t) for P, t in ptms[l % 2]]
+            while layer < g.layers[0] and deeper(layer) == active:
+                gates += self._window_gates(g, ptms, layer, active)
+                layer += self.window
             x = run_circuit(x[:active], gates, (4,) * n, self.k)
-            layer = end
-            nxt = active
-            while nxt > 0 and g.layers[nxt - 1] <= layer:
-                nxt -= 1
+            nxt = deeper(layer)
             for b in range(nxt, active):
                 finals[b] = x[b:b + 1]
             active = nxt
@@ -262,10 +311,11 @@ class CircuitBatch:
 
 
 def plain_forward_backward(circuit, weight, dtype=torch.float32, device=None,
-                           checkpoint_every=16):
-    """One circuit on its own (batch of 1, eager launches): the per-circuit
-    baseline the batched engine is compared with."""
+                           checkpoint_every=16, fuse_qudits=3):
+    """One circuit on its own (batch of 1, eager launches, the same gate
+    fusion and kernels): the per-circuit baseline the batched engine is
+    compared with."""
     device = device or torch.device('cuda', torch.cuda.current_device())
     b = CircuitBatch([circuit], dtype=dtype, device=device, checkpoint_every=checkpoint_every,
-                     persistent_kernel=False)
+                     persistent_kernel=False, fuse_qudits=fuse_qudits)
     return b.step([weight])

@@ -889,9 +889,15 @@ static bool use_fused(const Plan& p) {
 // target offsets are not additive over the bits of the PTM index, which
 // cannot happen for Pauli dimensions (perfect squares whose product is 64 or
 // 256 are powers of 4) but is checked rather than assumed.
+// batch > 1: states n_in / n_out elements apart, so vector accesses also
+// need those strides to keep 16-byte alignment.
 static bool make_fused_plan(const Plan& p, int elem_bytes, const void* in, const void* out,
-                            FusedPlan& fp) {
+                            FusedPlan& fp, uint32_t batch = 1) {
   if (!use_fused(p)) return false;
+  const bool in_al = reinterpret_cast<uintptr_t>(in) % 16 == 0 &&
+                     (batch <= 1 || (p.n_in * elem_bytes) % 16 == 0);
+  const bool out_al = reinterpret_cast<uintptr_t>(out) % 16 == 0 &&
+                      (batch <= 1 || (p.n_out * elem_bytes) % 16 == 0);
   const int D = p.din;
   for (int k = 0; k < D; ++k) {
     int64_t si = 0, so = 0;
@@ -903,8 +909,8 @@ static bool make_fused_plan(const Plan& p, int elem_bytes, const void* in, const
     if (si != p.off_in[k] || so != p.off_out[k]) return false;
   }
   const int VL = 16 / elem_bytes;
-  auto width = [&](const std::vector<int64_t>& off, const int64_t* gs, const void* ptr) {
-    if (reinterpret_cast<uintptr_t>(ptr) % 16) return 1;
+  auto width = [&](const std::vector<int64_t>& off, const int64_t* gs, bool aligned) {
+    if (!aligned) return 1;
     for (int e = 1; e < VL; ++e)
       if (off[e] != e) return 1;
     for (int b = VL; b < D; b <<= 1)
@@ -914,8 +920,8 @@ static bool make_fused_plan(const Plan& p, int elem_bytes, const void* in, const
     return VL;
   };
   fp.D = D;
-  fp.vi = width(p.off_in, p.g.gs_in, in);
-  fp.vo = width(p.off_out, p.g.gs_out, out);
+  fp.vi = width(p.off_in, p.g.gs_in, in_al);
+  fp.vo = width(p.off_out, p.g.gs_out, out_al);
   // float64 256 x 256: gathering vector stores out of the 64 accumulators
   // costs an extra 128 VGPRs (one wave per SIMD); scalar stores keep two
   if (D == 256 && elem_bytes == 8) fp.vo = 1;
@@ -924,9 +930,7 @@ static bool make_fused_plan(const Plan& p, int elem_bytes, const void* in, const
   // consecutive orbits (the planner's vector width already checks that every
   // stride and offset keeps such vectors aligned)
   fp.vn = 1;
-  if (D == 64 && fp.vi == 1 && fp.vo == 1 && p.vec > 1 &&
-      reinterpret_cast<uintptr_t>(in) % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0)
-    fp.vn = p.vec;
+  if (D == 64 && fp.vi == 1 && fp.vo == 1 && p.vec > 1 && in_al && out_al) fp.vn = p.vec;
   memset(&fp.offs, 0, sizeof(fp.offs));
   const int KS = D / 4;
   for (int u = 0; u < KS / fp.vi; ++u) fp.offs.in_u[u] = p.off_in[fused_k(fp.vi, fp.vi * u, 0)];
@@ -1215,9 +1219,9 @@ static int emulate_lds(const Plan& p, const LdsPlan& lp, int elem_bytes, const d
 // assumption), the fragment-ordered PTM reads, the MFMA contraction in
 // hardware index order and the accumulator stores.
 static int emulate_fused(const Plan& p, int elem_bytes, const double* ptm, const double* in,
-                         double* out) {
+                         double* out, uint32_t batch = 1) {
   FusedPlan fp;
-  QSV_CHECK_ARG(make_fused_plan(p, elem_bytes, nullptr, nullptr, fp),
+  QSV_CHECK_ARG(make_fused_plan(p, elem_bytes, nullptr, nullptr, fp, batch),
                 "fused: offsets not additive");
   const int D = fp.D, MB = D / 16, KS = D / 4;
   const int vi = fp.vi, vo = fp.vo, vn = fp.vn;
@@ -1377,13 +1381,23 @@ extern "C" int qsv_apply_ptm_batched_tiled(int dtype, const void* in, void* out,
   QSV_CHECK_ARG(batch >= 0 && batch <= 65535, "batch %d outside [0, 65535]", batch);
   QSV_CHECK_ARG(in && out && ptm_dev, "null pointer argument");
   QSV_CHECK_ARG(ptm_batch_stride >= 0, "negative PTM batch stride");
+  hipStream_t s = static_cast<hipStream_t>(stream);
   if (!fits_inline(p.din, p.dout)) {
-    set_error("batched apply supports PTMs up to %d x %d, got %d x %d", kInline, kInline,
-              p.dout, p.din);
+    // dense 64 x 64 (three qubits): MFMA kernel, each state's row-major PTM
+    // gathered into fragment order by the kernel itself
+    FusedPlan fp;
+    if (p.din == 64 && p.dout == 64 &&
+        make_fused_plan(p, dtype_size(dtype), in, out, fp, uint32_t(batch))) {
+      QSV_CHECK_ARG(!tile || tile->kernel == QSV_KERNEL_AUTO,
+                    "dense 64 x 64 PTMs take no launch shape");
+      return launch_fused64_batched(dtype, fp, p.g, p.nt, in, out, ptm_dev, ptm_batch_stride,
+                                    uint32_t(batch), s);
+    }
+    set_error("batched apply supports PTMs up to %d x %d and dense 64 x 64, got %d x %d",
+              kInline, kInline, p.dout, p.din);
     return QSV_ENOTSUP;
   }
   p.batch = uint32_t(batch);
-  hipStream_t s = static_cast<hipStream_t>(stream);
   const bool small = std::max(p.din, p.dout) <= 4;
   if (dtype == QSV_F64)
     return small ? run_ptr<double, 4>(p, ptm_dev, in, out, s, ptm_batch_stride, tile)
@@ -1403,11 +1417,20 @@ extern "C" int qsv_emulate_apply_ptm_tiled(int dtype, const double* in, double* 
   QSV_CHECK_ARG(in && out && ptm_f64, "null pointer argument");
   QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24) && p.n_out <= (int64_t(1) << 24),
                 "emulation is for small vectors");
+  const int es = dtype_size(dtype);
   if (!fits_inline(p.din, p.dout)) {
-    set_error("batched apply supports PTMs up to %d x %d", kInline, kInline);
+    // the batched 64 x 64 MFMA path (16-byte aligned states assumed, as
+    // torch allocations are; batch > 1 so the state strides count too)
+    FusedPlan fp;
+    if (p.din == 64 && p.dout == 64 && make_fused_plan(p, es, nullptr, nullptr, fp, 2)) {
+      QSV_CHECK_ARG(!tile || tile->kernel == QSV_KERNEL_AUTO,
+                    "dense 64 x 64 PTMs take no launch shape");
+      if (path_out) *path_out = 4 | (fp.vi << 4) | (fp.vo << 8) | (fp.vn << 12);
+      return emulate_fused(p, es, ptm_f64, in, out, 2);
+    }
+    set_error("batched apply supports PTMs up to %d x %d and dense 64 x 64", kInline, kInline);
     return QSV_ENOTSUP;
   }
-  const int es = dtype_size(dtype);
   LdsPlan lp;
   if (make_lds_plan(p, es, lp, tile)) {
     if (path_out) *path_out = 2;

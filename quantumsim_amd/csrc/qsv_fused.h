@@ -100,6 +100,14 @@ constexpr int fused_chunk_steps(int elem_bytes) { return elem_bytes == 8 ? 4 : 8
 int launch_fused(int dtype, const FusedPlan& fp, const Geom& g, bool nt, const void* in,
                  void* out, const void* ptm_frag, hipStream_t s);
 
+// 64 x 64 over `batch` states (g.bs_in / g.bs_out elements apart), state b
+// with the row-major device PTM ptm_rows + b * pstride (state's element
+// type; pstride 0 = one PTM for all), gathered into fragment order in LDS by
+// each block: no workspace, kernel launches only.
+int launch_fused64_batched(int dtype, const FusedPlan& fp, const Geom& g, bool nt,
+                           const void* in, void* out, const void* ptm_rows, int64_t pstride,
+                           uint32_t batch, hipStream_t s);
+
 // Packs a device PTM (row-major D x D, state's element type) into fragment
 // order at pf (D * D elements) on stream s; the device form of
 // fused_fragments.

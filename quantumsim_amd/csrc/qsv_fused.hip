@@ -21,6 +21,8 @@
 #include "qsv_fused.h"
 #include "qsv_mfma.h"
 
+#include <algorithm>
+
 namespace qsv {
 
 int fused_nb(int elem_bytes, int D, int vn) {
@@ -75,7 +77,8 @@ __host__ __device__ constexpr bool fused_nt_stores() {
 template <typename T, int VI, int VO, int VN, bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_fused64(const T* __restrict__ in, T* __restrict__ out, const Geom g,
-              const uint32_t n_orbits, const T* __restrict__ pf, const FusedOffs fo) {
+              const uint32_t n_orbits, const T* __restrict__ pf, const FusedOffs fo,
+              const int64_t pstride, const int prow) {
   constexpr int MB = 4, KS = 16;
   constexpr int NB = VN > 1 ? VN : (sizeof(T) == 4 ? 2 : 1);
   constexpr int NL = VN > 1 ? 1 : NB;  // orbit bases per lane
@@ -83,7 +86,19 @@ __global__ void __launch_bounds__(kBlock)
   typedef typename Mfma<T>::acc Acc;
   typedef typename VT<T, VN>::t VNv;
   __shared__ T As[KS * MB * 64];
-  for (int i = threadIdx.x; i < KS * MB * 64; i += kBlock) As[i] = pf[i];
+  // batch entry blockIdx.y: its state and its PTM (pstride elements apart;
+  // 0 = one PTM for all).  prow: the PTM is row-major and is gathered into
+  // fragment order here (batched device PTMs); otherwise pf is already in
+  // fragment order.
+  in += int64_t(blockIdx.y) * g.bs_in;
+  out += int64_t(blockIdx.y) * g.bs_out;
+  pf += int64_t(blockIdx.y) * pstride;
+  if (prow) {
+    for (int i = threadIdx.x; i < KS * MB * 64; i += kBlock)
+      As[i] = pf[fused_frag_source(64, int(sizeof(T)), VI, VO, i)];
+  } else {
+    for (int i = threadIdx.x; i < KS * MB * 64; i += kBlock) As[i] = pf[i];
+  }
   __syncthreads();
 
   const int lane = threadIdx.x & 63, n = lane & 15, j = lane >> 4;
@@ -266,17 +281,20 @@ static uint32_t resident_blocks(const void* kern) {
   return uint32_t(per_cu) * uint32_t(cus);
 }
 
+// Batched launches (batch > 1) split the resident blocks over the states;
+// every state gets at least one block.
 template <typename T, int VI, int VO, int VN, bool NT>
 static int launch64(const FusedPlan& fp, const Geom& g, const void* in, void* out, const void* pf,
-                    hipStream_t s) {
+                    hipStream_t s, uint32_t batch, int64_t pstride, int prow) {
   constexpr int NB = VN > 1 ? VN : (sizeof(T) == 4 ? 2 : 1);
   const auto kern = k_fused64<T, VI, VO, VN, NT>;
   static const uint32_t cap = resident_blocks(reinterpret_cast<const void*>(kern));
   const uint32_t ntiles = (fp.n_orbits + 16 * NB - 1) / (16 * NB);
   const uint32_t want = (ntiles + kBlock / 64 - 1) / (kBlock / 64);
-  hipLaunchKernelGGL(kern, dim3(want < cap ? want : cap), dim3(kBlock), 0, s,
+  const uint32_t per_state = std::max<uint32_t>(1, cap / std::max<uint32_t>(batch, 1));
+  hipLaunchKernelGGL(kern, dim3(want < per_state ? want : per_state, batch), dim3(kBlock), 0, s,
                      static_cast<const T*>(in), static_cast<T*>(out), g, fp.n_orbits,
-                     static_cast<const T*>(pf), fp.offs);
+                     static_cast<const T*>(pf), fp.offs, pstride, prow);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
 }
@@ -296,15 +314,16 @@ static int launch256(const FusedPlan& fp, const Geom& g, const void* in, void* o
 
 template <typename T, int VL, bool NT>
 static int dispatch(const FusedPlan& fp, const Geom& g, const void* in, void* out,
-                    const void* pf, hipStream_t s) {
+                    const void* pf, hipStream_t s, uint32_t batch = 1, int64_t pstride = 0,
+                    int prow = 0) {
   const int key = (fp.vi == VL ? 2 : 0) | (fp.vo == VL ? 1 : 0);
   if (fp.D == 64) {
-    if (fp.vn > 1) return launch64<T, 1, 1, VL, NT>(fp, g, in, out, pf, s);
+    if (fp.vn > 1) return launch64<T, 1, 1, VL, NT>(fp, g, in, out, pf, s, batch, pstride, prow);
     switch (key) {
-      case 0: return launch64<T, 1, 1, 1, NT>(fp, g, in, out, pf, s);
-      case 1: return launch64<T, 1, VL, 1, NT>(fp, g, in, out, pf, s);
-      case 2: return launch64<T, VL, 1, 1, NT>(fp, g, in, out, pf, s);
-      default: return launch64<T, VL, VL, 1, NT>(fp, g, in, out, pf, s);
+      case 0: return launch64<T, 1, 1, 1, NT>(fp, g, in, out, pf, s, batch, pstride, prow);
+      case 1: return launch64<T, 1, VL, 1, NT>(fp, g, in, out, pf, s, batch, pstride, prow);
+      case 2: return launch64<T, VL, 1, 1, NT>(fp, g, in, out, pf, s, batch, pstride, prow);
+      default: return launch64<T, VL, VL, 1, NT>(fp, g, in, out, pf, s, batch, pstride, prow);
     }
   }
   switch (key) {
@@ -340,6 +359,21 @@ int launch_fused_pack(int dtype, const FusedPlan& fp, const void* ptm_dev, void*
                        fp.vo);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
+}
+
+int launch_fused64_batched(int dtype, const FusedPlan& fp, const Geom& g, bool nt,
+                           const void* in, void* out, const void* ptm_rows, int64_t pstride,
+                           uint32_t batch, hipStream_t s) {
+  if (fp.D != 64) {
+    set_error("internal: batched MFMA apply is 64 x 64 only, got D = %d", fp.D);
+    return QSV_ENOTSUP;
+  }
+  if (fp.n_orbits == 0 || batch == 0) return QSV_OK;
+  if (dtype == QSV_F64)
+    return nt ? dispatch<double, 2, true>(fp, g, in, out, ptm_rows, s, batch, pstride, 1)
+              : dispatch<double, 2, false>(fp, g, in, out, ptm_rows, s, batch, pstride, 1);
+  return nt ? dispatch<float, 4, true>(fp, g, in, out, ptm_rows, s, batch, pstride, 1)
+            : dispatch<float, 4, false>(fp, g, in, out, ptm_rows, s, batch, pstride, 1);
 }
 
 int launch_fused(int dtype, const FusedPlan& fp, const Geom& g, bool nt, const void* in,

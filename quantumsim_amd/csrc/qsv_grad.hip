@@ -30,8 +30,8 @@ constexpr int kGradBlocks = 256;
 constexpr int kGradUnroll = 4;
 
 struct GradOffs {
-  int64_t off_in[16];
-  int64_t off_out[16];
+  int64_t off_in[64];
+  int64_t off_out[64];
 };
 
 template <typename T, int VN>
@@ -97,6 +97,107 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// PTMs of 17..64 rows or columns (dense three-qubit blocks): the same
+// contraction with a 64 x 64 output, i.e. 4 x 4 MFMA tiles per wave.  Lane
+// (c, k) loads entries c, 16 + c, 32 + c, 48 + c of ybar (the A operands of
+// the four row blocks) and of x (the B operands of the four column blocks)
+// of its orbit; each k-step is 16 MFMAs.  Float64 states use
+// v_mfma_f64_16x16x4f64; float32 states v_mfma_f32_16x16x4f32 (float32 sums
+// within a wave, i.e. over at most a few thousand orbits; 64 float64 MFMAs
+// per 16 orbits would make the kernel MFMA-bound below the HBM rate).  The
+// four waves' tiles are summed in float64 in LDS in a fixed order (32 KB),
+// one row block at a time, then the block writes its partial tile.
+constexpr int kGrad64Blocks = 64;
+constexpr int kGrad64Unroll = 1;
+
+template <typename T, int VN>
+__global__ void __launch_bounds__(kBlock)
+    k_ptm_grad64(const T* __restrict__ x, const T* __restrict__ ybar, const Geom g,
+                 const GradOffs offs, const uint32_t n_orbits, double* __restrict__ partial) {
+  typedef typename Mfma<T>::acc Acc;
+  typedef typename VT<T, VN>::t Vec;
+  __shared__ double red[64 * 64];
+  const uint32_t b = blockIdx.y;
+  x += int64_t(b) * g.bs_in;
+  ybar += int64_t(b) * g.bs_out;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c = lane & 15, k = lane >> 4;
+  int64_t oi[4], oo[4];
+  bool vi[4], vo[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    vi[q] = 16 * q + c < g.din;
+    vo[q] = 16 * q + c < g.dout;
+    oi[q] = vi[q] ? offs.off_in[16 * q + c] : 0;
+    oo[q] = vo[q] ? offs.off_out[16 * q + c] : 0;
+  }
+  Acc acc[4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = Acc(0);
+  constexpr uint32_t STEP = 4 * VN;
+  constexpr int U = kGrad64Unroll;
+  const uint32_t nsteps = (n_orbits + STEP - 1) / STEP;
+  const uint32_t nchunks = (nsteps + U - 1) / U;
+  for (uint32_t tc = blockIdx.x * (kBlock / 64) + w; tc < nchunks;
+       tc += gridDim.x * (kBlock / 64)) {
+    Vec xv[U][4], yv[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t m = (tc * U + u) * STEP + VN * k;
+      const bool ok = m < n_orbits;  // VN divides n_orbits: whole vectors
+      int64_t bin, bout;
+      orbit_base(g, ok ? m : 0, bin, bout);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const bool lx = ok && vi[q], ly = ok && vo[q];
+        xv[u][q] = *reinterpret_cast<const Vec*>(x + (lx ? bin + oi[q] : 0));
+        yv[u][q] = *reinterpret_cast<const Vec*>(ybar + (ly ? bout + oo[q] : 0));
+        if (!lx) xv[u][q] = Vec(0);
+        if (!ly) yv[u][q] = Vec(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < VN; ++e)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            if constexpr (VN == 1)
+              acc[rb][cb] = Mfma<T>::op(yv[u][rb], xv[u][cb], acc[rb][cb]);
+            else
+              acc[rb][cb] = Mfma<T>::op(yv[u][rb][e], xv[u][cb][e], acc[rb][cb]);
+          }
+  }
+  // slot r of lane (c, k) in tile (rb, cb) holds entry (16 rb + row, 16 cb + c)
+  // with row = k + 4 r (float64 MFMA) or 4 k + r (float32 MFMA)
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+    for (int ww = 0; ww < kBlock / 64; ++ww) {
+      if (w == ww) {
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = sizeof(T) == 8 ? k + 4 * r : 4 * k + r;
+            const int idx = (16 * rb + row) * 64 + 16 * cb + c;
+            red[idx] = (ww == 0 ? 0.0 : red[idx]) + double(acc[rb][cb][r]);
+          }
+      }
+      __syncthreads();
+    }
+  }
+  const int dd = g.dout * g.din;
+  for (int j = tid; j < dd; j += kBlock) {
+    const int o = j / g.din, i = j - o * g.din;
+    partial[(int64_t(b) * gridDim.x + blockIdx.x) * dd + j] = red[o * 64 + i];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_ptm_grad_final(const double* __restrict__ partial, int nblocks, int dd,
@@ -157,7 +258,7 @@ static int make_grad_plan(int elem_bytes, int rank, const int64_t* dims, int k,
     din *= dims[targets[j]];
     dout *= dims_out_t[j];
   }
-  QSV_CHECK_ARG(din <= 16 && dout <= 16, "PTM gradient supports up to 16 x 16, got %lld x %lld",
+  QSV_CHECK_ARG(din <= 64 && dout <= 64, "PTM gradient supports up to 64 x 64, got %lld x %lld",
                 (long long)dout, (long long)din);
   p.din = int(din);
   p.dout = int(dout);
@@ -237,6 +338,20 @@ static int grad_blocks(uint32_t n_orbits, int batch) {
   return int(b < 1 ? 1 : (b > kGradBlocks ? kGradBlocks : b));
 }
 
+// Larger than 16 x 16: 16 times the MFMA work per orbit, 32 KB partial
+// tiles; at least 512 orbits per block, ~2048 blocks over the batch, at
+// most kGrad64Blocks per state.
+static bool grad_wide(const GradPlan& p) { return p.din > 16 || p.dout > 16; }
+
+static int grad64_blocks(uint32_t n_orbits, int batch) {
+  uint32_t b = (n_orbits + 511) / 512;
+  const uint32_t fill = uint32_t((2048 + std::max(batch, 1) - 1) / std::max(batch, 1));
+  b = std::min(b, fill);
+  return int(b < 1 ? 1 : (b > kGrad64Blocks ? kGrad64Blocks : b));
+}
+
+static int max_grad_blocks(const GradPlan& p) { return grad_wide(p) ? kGrad64Blocks : kGradBlocks; }
+
 }  // namespace qsv
 
 using namespace qsv;
@@ -247,7 +362,7 @@ extern "C" int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64
   GradPlan p;
   const int rc = make_grad_plan(8, rank, dims, k, targets, dims_out_t, p);
   if (rc) return rc;
-  return int64_t(batch) * kGradBlocks * p.din * p.dout * 8;
+  return int64_t(batch) * max_grad_blocks(p) * p.din * p.dout * 8;
 }
 
 template <typename T>
@@ -255,6 +370,15 @@ static void launch_grad(const GradPlan& p, int vn, dim3 grid, const void* x, con
                         double* partial, hipStream_t s) {
   const T* xx = static_cast<const T*>(x);
   const T* yy = static_cast<const T*>(ybar);
+  if (grad_wide(p)) {
+    if (vn == 16 / int(sizeof(T)))
+      hipLaunchKernelGGL((k_ptm_grad64<T, 16 / sizeof(T)>), grid, dim3(kBlock), 0, s, xx, yy, p.g,
+                         p.offs, p.n_orbits, partial);
+    else
+      hipLaunchKernelGGL((k_ptm_grad64<T, 1>), grid, dim3(kBlock), 0, s, xx, yy, p.g, p.offs,
+                         p.n_orbits, partial);
+    return;
+  }
   if (vn == 16 / int(sizeof(T)))
     hipLaunchKernelGGL((k_ptm_grad<T, 16 / sizeof(T)>), grid, dim3(kBlock), 0, s, xx, yy, p.g,
                        p.offs, p.n_orbits, partial);
@@ -285,8 +409,13 @@ extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, in
   if (rc) return rc;
   if (batch == 0) return QSV_OK;
   QSV_CHECK_ARG(x && ybar && dptm, "null pointer argument");
-  const int nb = tile && tile->grad_blocks > 0 ? tile->grad_blocks
-                                               : grad_blocks(p.n_orbits, batch);
+  QSV_CHECK_ARG(!tile || tile->grad_blocks <= max_grad_blocks(p),
+                "tile: grad_blocks %d above %d for a %d x %d PTM", tile ? tile->grad_blocks : 0,
+                max_grad_blocks(p), p.dout, p.din);
+  const int nb = tile && tile->grad_blocks > 0
+                     ? tile->grad_blocks
+                     : (grad_wide(p) ? grad64_blocks(p.n_orbits, batch)
+                                     : grad_blocks(p.n_orbits, batch));
   const int64_t need = int64_t(batch) * nb * p.din * p.dout * 8;
   QSV_CHECK_ARG(workspace && workspace_bytes >= need, "PTM gradient needs %lld workspace bytes",
                 (long long)need);
@@ -314,6 +443,106 @@ extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, in
   return QSV_OK;
 }
 
+// Host replay of k_ptm_grad64 + k_ptm_grad_final for one state (same checks
+// as the replay of the 16 x 16 kernel below).
+static int emulate_grad64(const GradPlan& p, int es, const double* x, const double* ybar, int nb,
+                          double* dptm) {
+  const int VN = p.vn, waves = kBlock / 64, U = kGrad64Unroll;
+  const uint32_t STEP = 4 * VN, nsteps = (p.n_orbits + STEP - 1) / STEP;
+  const uint32_t nchunks = (nsteps + U - 1) / U;
+  const int dd = p.din * p.dout;
+  std::vector<int> used(p.n_orbits, 0);
+  std::vector<double> partial(size_t(nb) * dd, 0.0);
+  // acc[w][rb][cb][lane][r]
+  std::vector<double> acc(size_t(waves) * 16 * 64 * 4);
+  std::vector<double> A(size_t(4) * 64 * 16), B(size_t(4) * 64 * 16);  // [q][lane][e]
+  for (int blk = 0; blk < nb; ++blk) {
+    std::fill(acc.begin(), acc.end(), 0.0);
+    for (int w = 0; w < waves; ++w)
+      for (uint32_t tc = blk * waves + w; tc < nchunks; tc += nb * waves)
+        for (uint32_t t = tc * U; t < tc * U + U; ++t) {
+          for (int lane = 0; lane < 64; ++lane) {
+            const int c = lane & 15, kk = lane >> 4;
+            const uint32_t m = t * STEP + VN * kk;
+            const bool ok = m < p.n_orbits;
+            int64_t bin, bout;
+            orbit_base(p.g, ok ? m : 0, bin, bout);
+            if (ok && c == 0) {
+              for (int ee = 0; ee < VN; ++ee) {
+                QSV_CHECK_ARG(m + ee < p.n_orbits, "grad64: vector past the last orbit");
+                int64_t bi2, bo2;
+                orbit_base(p.g, m + ee, bi2, bo2);
+                QSV_CHECK_ARG(bi2 == bin + ee && bo2 == bout + ee,
+                              "grad64: orbits of a vector not adjacent");
+                ++used[m + ee];
+              }
+            }
+            for (int q = 0; q < 4; ++q)
+              for (int e = 0; e < VN; ++e) {
+                double& a = A[(size_t(q) * 64 + lane) * 16 + e];
+                double& bb = B[(size_t(q) * 64 + lane) * 16 + e];
+                a = bb = 0.0;
+                if (!ok) continue;
+                const int ent = 16 * q + c;
+                if (ent < p.din) {
+                  const int64_t ad = bin + p.offs.off_in[ent] + e;
+                  QSV_CHECK_ARG((ad - e) % VN == 0, "grad64: misaligned vector load");
+                  QSV_CHECK_ARG(ad >= 0 && ad < p.n_in, "grad64: x load %lld out of bounds",
+                                (long long)ad);
+                  bb = x[ad];
+                }
+                if (ent < p.dout) {
+                  const int64_t ad = bout + p.offs.off_out[ent] + e;
+                  QSV_CHECK_ARG((ad - e) % VN == 0, "grad64: misaligned vector load");
+                  QSV_CHECK_ARG(ad >= 0 && ad < p.n_out, "grad64: ybar load %lld out of bounds",
+                                (long long)ad);
+                  a = ybar[ad];
+                }
+              }
+          }
+          // tile (rb, cb): D[row][col] += sum_kk A_rb[row + 16 kk] B_cb[col + 16 kk];
+          // slot r of lane l holds column l % 16 and row (l / 16) + 4 r
+          // (float64 MFMA) or 4 (l / 16) + r (float32 MFMA)
+          for (int e = 0; e < VN; ++e)
+            for (int rb = 0; rb < 4; ++rb)
+              for (int cb = 0; cb < 4; ++cb)
+                for (int lane = 0; lane < 64; ++lane)
+                  for (int r = 0; r < 4; ++r) {
+                    const int row = es == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+                    const int col = lane & 15;
+                    double d = 0.0;
+                    for (int kk = 0; kk < 4; ++kk)
+                      d += A[(size_t(rb) * 64 + row + 16 * kk) * 16 + e] *
+                           B[(size_t(cb) * 64 + col + 16 * kk) * 16 + e];
+                    acc[((((size_t(w) * 4 + rb) * 4 + cb) * 64) + lane) * 4 + r] += d;
+                  }
+        }
+    std::vector<double> red(64 * 64, 0.0);
+    for (int w = 0; w < waves; ++w)
+      for (int rb = 0; rb < 4; ++rb)
+        for (int cb = 0; cb < 4; ++cb)
+          for (int lane = 0; lane < 64; ++lane)
+            for (int r = 0; r < 4; ++r) {
+              const int row = es == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+              const int idx = (16 * rb + row) * 64 + 16 * cb + (lane & 15);
+              const double v = acc[((((size_t(w) * 4 + rb) * 4 + cb) * 64) + lane) * 4 + r];
+              red[idx] = (w == 0 ? 0.0 : red[idx]) + v;
+            }
+    for (int j = 0; j < dd; ++j) {
+      const int o = j / p.din, i = j - o * p.din;
+      partial[size_t(blk) * dd + j] = red[o * 64 + i];
+    }
+  }
+  for (uint32_t m = 0; m < p.n_orbits; ++m)
+    QSV_CHECK_ARG(used[m] == 1, "grad64: orbit %u consumed %d times", m, used[m]);
+  for (int j = 0; j < dd; ++j) {
+    double s = 0.0;
+    for (int blk = 0; blk < nb; ++blk) s += partial[size_t(blk) * dd + j];
+    dptm[j] = s;
+  }
+  return QSV_OK;
+}
+
 // Host replay of k_ptm_grad + k_ptm_grad_final for one state: every lane's
 // loads (bounds and vector alignment checked), the MFMA contraction in
 // hardware index order (A = ybar, B = x, k = the four orbits of a k-step),
@@ -330,8 +559,14 @@ extern "C" int qsv_emulate_ptm_grad(int dtype, const double* x, const double* yb
   QSV_CHECK_ARG(x && ybar && dptm, "null pointer argument");
   QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24) && p.n_out <= (int64_t(1) << 24),
                 "emulation is for small vectors");
-  QSV_CHECK_ARG(grad_blocks_forced >= 0 && grad_blocks_forced <= kGradBlocks,
-                "grad_blocks outside [0, %d]", kGradBlocks);
+  QSV_CHECK_ARG(grad_blocks_forced >= 0 && grad_blocks_forced <= max_grad_blocks(p),
+                "grad_blocks outside [0, %d]", max_grad_blocks(p));
+  if (vn_out) *vn_out = p.vn;
+  if (grad_wide(p))
+    return emulate_grad64(p, dtype_size(dtype), x, ybar,
+                          grad_blocks_forced > 0 ? grad_blocks_forced
+                                                 : grad64_blocks(p.n_orbits, 1),
+                          dptm);
   const int VN = p.vn, waves = kBlock / 64;
   const int nb = grad_blocks_forced > 0 ? grad_blocks_forced : grad_blocks(p.n_orbits, 1);
   if (vn_out) *vn_out = VN;

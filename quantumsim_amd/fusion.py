@@ -18,17 +18,21 @@ Square PTMs (same input and output dimension per qudit) only.
 import itertools
 import numpy as np
 
-__all__ = ['fuse', 'FusedGate', 'embed_ptm', 'apply_circuit']
+__all__ = ['fuse', 'fuse_plan', 'FusedGate', 'embed_ptm', 'embed_ptm_batched', 'fuse_batched',
+           'apply_circuit']
 
 
 class FusedGate:
-    """A dense PTM over ``qudits`` (ascending), layout (out..., in...)."""
-    __slots__ = ('ptm', 'qudits', 'n_source')
+    """A dense PTM over ``qudits`` (ascending), layout (out..., in...);
+    ``sources``: indices of the input gates it is the product of, in
+    application order."""
+    __slots__ = ('ptm', 'qudits', 'n_source', 'sources')
 
-    def __init__(self, ptm, qudits, n_source):
+    def __init__(self, ptm, qudits, n_source, sources=()):
         self.ptm = ptm
         self.qudits = tuple(qudits)
         self.n_source = n_source
+        self.sources = tuple(sources)
 
     def __iter__(self):          # (ptm, qudits) unpacking, like the inputs
         yield self.ptm
@@ -54,12 +58,13 @@ def embed_ptm(ptm, qudits, block, dims):
 
 
 class _Block:
-    __slots__ = ('qudits', 'mat', 'n')
+    __slots__ = ('qudits', 'mat', 'n', 'src')
 
-    def __init__(self, qudits, mat, n):
+    def __init__(self, qudits, mat, n, src):
         self.qudits = qudits      # sorted tuple
         self.mat = mat            # D x D in the block's qudit order
         self.n = n
+        self.src = src            # source gate indices, application order
 
 
 def _merge(blocks, dims):
@@ -69,7 +74,7 @@ def _merge(blocks, dims):
     for b in blocks:
         mat = embed_ptm(b.mat.reshape([dims[q] for q in b.qudits] * 2), b.qudits, qs,
                         dims) @ mat
-    return _Block(qs, mat, sum(b.n for b in blocks))
+    return _Block(qs, mat, sum(b.n for b in blocks), [i for b in blocks for i in b.src])
 
 
 def fuse(gates, dims, max_qudits=2):
@@ -81,11 +86,11 @@ def fuse(gates, dims, max_qudits=2):
 
     def emit(b):
         shape = [dims[q] for q in b.qudits] * 2
-        out.append(FusedGate(b.mat.reshape(shape), b.qudits, b.n))
+        out.append(FusedGate(b.mat.reshape(shape), b.qudits, b.n, b.src))
         for q in b.qudits:
             del open_blocks[q]
 
-    for ptm, qudits in gates:
+    for gi, (ptm, qudits) in enumerate(gates):
         qudits = tuple(int(q) for q in qudits)
         p = np.asarray(ptm)
         k = len(qudits)
@@ -120,9 +125,9 @@ def fuse(gates, dims, max_qudits=2):
             if base.qudits != qs:
                 prev = embed_ptm(prev.reshape([dims[q] for q in base.qudits] * 2),
                                  base.qudits, qs, dims)
-            nb = _Block(qs, embed_ptm(p, qudits, qs, dims) @ prev, base.n + 1)
+            nb = _Block(qs, embed_ptm(p, qudits, qs, dims) @ prev, base.n + 1, base.src + [gi])
         else:
-            nb = _Block(qs, embed_ptm(p, qudits, qs, dims), 1)
+            nb = _Block(qs, embed_ptm(p, qudits, qs, dims), 1, [gi])
         for q in nb.qudits:
             open_blocks[q] = nb
     seen = []
@@ -131,6 +136,65 @@ def fuse(gates, dims, max_qudits=2):
             seen.append(b)
     for b in seen:
         emit(b)
+    return out
+
+
+_PLANS = {}
+_EYES = {}
+
+
+def fuse_plan(qudit_sets, dims, max_qudits=3):
+    """The blocks ``fuse`` forms for gates on ``qudit_sets`` (the grouping
+    depends only on which qudits the gates touch): [(block qudits, source
+    gate indices in application order), ...].  Cached."""
+    key = (tuple(tuple(int(q) for q in t) for t in qudit_sets), tuple(dims), max_qudits)
+    plan = _PLANS.get(key)
+    if plan is None:
+        probe = [(np.eye(int(np.prod([dims[q] for q in t]))).reshape([dims[q] for q in t] * 2), t)
+                 for t in key[0]]
+        plan = [(f.qudits, f.sources) for f in fuse(probe, dims, max_qudits)]
+        _PLANS[key] = plan
+    return plan
+
+
+def embed_ptm_batched(ptm, qudits, block, dims):
+    """Torch form of ``embed_ptm`` for a batch: ptm [B, D, D] acting on
+    ``qudits`` -> [B, Db, Db] over ``block`` (sorted), identity elsewhere."""
+    import torch
+    qudits, block = tuple(qudits), tuple(block)
+    if qudits == block:
+        return ptm
+    k, m = len(qudits), len(block)
+    pos = [block.index(q) for q in qudits]
+    B = ptm.shape[0]
+    p = ptm.reshape([B] + [dims[q] for q in qudits] * 2)
+    # labels: batch 0, outputs 1..m, inputs m+1..2m
+    ops = [p, [0] + [1 + pos[j] for j in range(k)] + [1 + m + pos[j] for j in range(k)]]
+    for b in range(m):
+        if b not in pos:
+            key = (dims[block[b]], ptm.dtype, ptm.device)
+            eye = _EYES.get(key)
+            if eye is None:
+                eye = _EYES[key] = torch.eye(key[0], dtype=ptm.dtype, device=ptm.device)
+            ops += [eye, [1 + b, 1 + m + b]]
+    out = torch.einsum(*ops, list(range(2 * m + 1)))
+    D = int(np.prod([dims[q] for q in block]))
+    return out.reshape(B, D, D)
+
+
+def fuse_batched(gates, dims, max_qudits=3):
+    """Fuse batched gates (objects with ``.ptm`` [B or 1, D, D] torch
+    tensors and ``.targets``) into dense blocks of up to ``max_qudits``
+    qudits with torch ops, so autograd reaches the source PTMs.  Returns
+    [(ptm [B, Db, Db], block qudits), ...] in application order."""
+    plan = fuse_plan([g.targets for g in gates], dims, max_qudits)
+    out = []
+    for block, src in plan:
+        mat = None
+        for i in src:
+            e = embed_ptm_batched(gates[i].ptm, gates[i].targets, block, dims)
+            mat = e if mat is None else e @ mat
+        out.append((mat, block))
     return out
 
 

@@ -207,16 +207,23 @@ class TrainableCircuit:
         mk = lambda a: torch.tensor(a, dtype=dtype, device=device)
         return mk(ang), mk(t1), mk(t2)
 
-    def gates(self, ang, t1, t2):
+    def gates(self, ang, t1, t2, fuse_qudits=0):
+        """The circuit's gates (per layer: workloads.fused_layer_gates);
+        with fuse_qudits >= 3 composed into dense blocks of up to that many
+        qubits (fusion.fuse_batched: differentiable torch products)."""
         rot = rotation_ptms(ang)                   # [S, blocks, n, 4, 4]
         damp = damping_ptms(t1, t2)                # [S, n, 4, 4]
         out = []
         for layer in range(self.layers):
             b = (layer // self.block) % self.n_blocks
             out += fused_layer_gates(self.n, layer, rot[:, b], damp)
+        if fuse_qudits and fuse_qudits > 2:
+            from .autograd import Gate
+            from .fusion import fuse_batched
+            out = [Gate(m, q) for m, q in fuse_batched(out, [4] * self.n, fuse_qudits)]
         return out
 
-    def populations(self, ang, t1, t2, dtype, checkpoint_every=8):
+    def populations(self, ang, t1, t2, dtype, checkpoint_every=8, fuse_qudits=0):
         import torch
         from .autograd import run_circuit, diagonal
         from .bases import general
@@ -224,7 +231,7 @@ class TrainableCircuit:
         N = 4 ** self.n
         x0 = torch.zeros((S, N), dtype=dtype, device=ang.device)
         x0[:, 0] = 1
-        gates = self.gates(ang, t1, t2)
+        gates = self.gates(ang, t1, t2, fuse_qudits)
         x = run_circuit(x0, gates, (4,) * self.n, checkpoint_every)
         return diagonal(x, [general(2)] * self.n)
 

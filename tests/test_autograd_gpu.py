@@ -242,3 +242,134 @@ def test_persistent_kernel_matches_batched_launches(dtype, tol):
     for a, b in zip(grads, grads_r):
         scale = b.abs().max().item() + 1e-30
         assert (a - b).abs().max().item() < (1e-10 if dtype == torch.float64 else 1e-4) * scale
+
+
+# ------------------------------------------------ dense 64 x 64 (three-qubit) paths
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+@pytest.mark.parametrize('targets', [(1, 2, 3), (2, 3, 4), (4, 0, 2), (0, 1, 2)])
+def test_apply_batched_dense64_matches_single(targets, dtype):
+    """Batched 64 x 64 apply (MFMA, per-state row-major PTMs gathered in the
+    kernel) equals one PauliVectorHIP per state (host PTM in fragment
+    order) bitwise; a shared PTM (stride 0) too."""
+    rng = np.random.default_rng(31)
+    n, B = 5, 5
+    print('dense64 batched', targets, dtype, flush=True)
+    x = rng.standard_normal((B, 4 ** n))
+    P = rng.standard_normal((B, 64, 64)) / 8
+    xt = torch.tensor(x, dtype=dtype, device=DEV)
+    y = apply(xt, torch.tensor(P, dtype=dtype, device=DEV), (4,) * n, targets).cpu()
+    ys = apply(xt, torch.tensor(P[:1], dtype=dtype, device=DEV), (4,) * n, targets).cpu()
+    for b in range(B):
+        for PP, yy in ((P[b], y), (P[0], ys)):
+            v = qa.PauliVectorHIP([bases.general(2)] * n, x[b].reshape((4,) * n), dtype=dtype)
+            v.apply_ptm(PP.reshape((4,) * 6), *targets)
+            assert torch.equal(yy[b], v.to_torch().reshape(-1).cpu()), b
+
+
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12), (torch.float32, 1e-5)])
+@pytest.mark.parametrize('targets', [(1, 2, 3), (7, 8, 9), (9, 0, 4), (0, 1)])
+def test_ptm_grad64_kernel_matches_einsum(targets, dtype, tol):
+    """The wide gradient kernel (dense 3-qubit PTMs; a 16 x 16 case for
+    comparison) on 10-qubit states, batch 3, against numpy."""
+    from quantumsim_amd.autograd import _ptm_grad_raw
+    rng = np.random.default_rng(6)
+    n, B = 10, 3
+    shape = (4,) * n
+    x = rng.standard_normal((B,) + shape)
+    yb = rng.standard_normal((B,) + shape)
+    if dtype == torch.float32:
+        x, yb = x.astype(np.float32).astype(np.float64), yb.astype(np.float32).astype(np.float64)
+    print('grad64', targets, dtype, flush=True)
+    got = _ptm_grad_raw(torch.tensor(x.reshape(B, -1), dtype=dtype, device=DEV),
+                        torch.tensor(yb.reshape(B, -1), dtype=dtype, device=DEV), shape,
+                        targets, tuple(4 for _ in targets)).double().cpu().numpy()
+    k = len(targets)
+    lx = [n + 1] + list(range(n))
+    ly = [n + 1] + list(range(n))
+    for j, t in enumerate(targets):
+        ly[1 + t] = n + 2 + j
+    ref = np.einsum(yb, ly, x, lx, [n + 1] + [n + 2 + j for j in range(k)] + list(targets))
+    ref = ref.reshape(B, 4 ** k, 4 ** k)
+    err = np.max(np.abs(got.reshape(ref.shape) - ref)) / np.max(np.abs(ref))
+    print(targets, dtype, 'rel err %.3e' % err)
+    assert err < tol
+
+
+def test_apply_dense64_gradcheck():
+    rng = np.random.default_rng(32)
+    n, B = 4, 2
+    x = torch.tensor(rng.standard_normal((B, 4 ** n)), device=DEV, requires_grad=True)
+    P = torch.tensor(rng.standard_normal((B, 64, 64)) / 8, device=DEV, requires_grad=True)
+    for targets in [(1, 2, 3), (3, 0, 1)]:
+        assert torch.autograd.gradcheck(lambda a, b: apply(a, b, (4,) * n, targets), (x, P),
+                                        eps=1e-6, atol=1e-8, rtol=1e-6)
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_fused_batch_equals_unfused(dtype):
+    """Circuits of 8-9 qubits (batched launches) with uneven depths: the
+    two-layer windows fused into 3-qubit blocks (identity PTMs for circuits
+    ending inside a window) give the populations and gradients of the
+    gate-by-gate batch; HIP-graph replays equal the eager step."""
+    circuits = random_circuit_batch(6, seed=41, n_range=(8, 9), layer_range=(3, 8))
+    assert len(set(c['layers'] for c in circuits)) > 2
+    fused = CircuitBatch(circuits, dtype=dtype, checkpoint_every=4, fuse_qudits=3)
+    plain = CircuitBatch(circuits, dtype=dtype, checkpoint_every=4, fuse_qudits=0)
+    assert not fused.small
+    w = fused.random_weights(5)
+    _, pf, gf = fused.step(w)
+    _, pp, gp = plain.step(w)
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    for a, b in zip(pf, pp):
+        assert (a - b).abs().max().item() < tol
+    for a, b in zip(gf, gp):
+        scale = b.abs().max().item() + 1e-30
+        assert (a - b).abs().max().item() < (1e-10 if dtype == torch.float64 else 1e-4) * scale
+    fused.capture(w)
+    # capture() tunes the launch shapes: an eager step with the tuned shapes
+    # is the reference for the replays (float32 gradients of dense blocks
+    # sum in float32 within a wave, so other block counts round differently)
+    _, pe, ge = fused.step(w)
+    for a, b in zip(ge, gf):
+        scale = b.abs().max().item() + 1e-30
+        assert (a - b).abs().max().item() < (1e-12 if dtype == torch.float64 else 1e-5) * scale
+    for rep in range(3):
+        torch.randn(1000, device=DEV).sum().item()
+        _, pg, gg = fused.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(pg, pe):
+            assert torch.allclose(a, b, rtol=1e-12, atol=1e-14), rep
+        for a, b in zip(gg, ge):
+            assert torch.allclose(a, b, rtol=1e-6, atol=1e-9), rep
+
+
+def test_fused_batch_matches_oracle():
+    """One fused 8-qubit circuit vs the oracle applying the unfused layer
+    PTMs (float64)."""
+    circuits = random_circuit_batch(2, seed=43, n_range=(8, 8), layer_range=(5, 7))
+    cb = CircuitBatch(circuits, dtype=torch.float64, checkpoint_every=4)
+    with torch.no_grad():
+        pops = cb.populations()
+    for gi, g in enumerate(cb.groups):
+        for b, i in enumerate(g.members):
+            ref = _oracle_circuit_pops(circuits[i])
+            assert np.max(np.abs(pops[gi][b].cpu().numpy() - ref)) < 1e-12
+
+
+def test_trainable_circuit_fused_gradient():
+    """The 12-qubit trainable circuit (reduced depth and sets) fused into
+    3-qubit blocks: loss and gradients equal the unfused run (float64)."""
+    tc = TrainableCircuit(n=8, layers=10)
+    ang, t1, t2 = tc.initial_params(3, seed=8, device=DEV, dtype=torch.float64)
+    with torch.no_grad():
+        target = tc.populations(ang + 0.1, t1, t2, torch.float64)
+    out = []
+    for fq in (0, 3):
+        params = [ang.clone().requires_grad_(), t1.clone().requires_grad_(),
+                  t2.clone().requires_grad_()]
+        p = tc.populations(*params, torch.float64, checkpoint_every=4, fuse_qudits=fq)
+        loss = tc.kl(target, p).sum()
+        out.append((loss.item(), torch.autograd.grad(loss, params)))
+    assert abs(out[0][0] - out[1][0]) < 1e-12
+    for a, b in zip(out[0][1], out[1][1]):
+        assert torch.allclose(a, b, rtol=1e-10, atol=1e-12)

@@ -86,3 +86,64 @@ def test_embed_ptm_identity_and_order():
 def test_fuse_rejects_nonsquare():
     with pytest.raises(ValueError):
         fuse([(np.zeros((3, 4)), (0,))], [4, 4])
+
+
+def test_fuse_batched_matches_host_fusion_per_batch_entry():
+    """The torch composition along fuse_plan equals fuse() of each batch
+    entry's PTMs; shared [1, D, D] PTMs broadcast; sequential application of
+    the blocks equals the gate list (oracle)."""
+    import torch
+    from quantumsim_amd.fusion import fuse_batched, fuse_plan
+
+    class G:
+        def __init__(self, ptm, targets):
+            self.ptm, self.targets = ptm, targets
+
+    rng = np.random.default_rng(7)
+    dims = [4, 4, 4, 4, 4]
+    B = 3
+    prog = []
+    for t in [(0, 1), (2,), (2, 3), (4,), (1, 2), (3, 4), (0,), (1,), (2, 1), (4, 3)]:
+        D = 4 ** len(t)
+        shared = len(prog) % 3 == 0
+        prog.append(G(torch.tensor(rng.standard_normal((1 if shared else B, D, D)) / D),
+                      t))
+    for k in (2, 3):
+        blocks = fuse_batched(prog, dims, k)
+        assert [b for _, b in blocks] == [q for q, _ in fuse_plan([g.targets for g in prog],
+                                                                   dims, k)]
+        for b in range(B):
+            host = [(g.ptm[min(b, g.ptm.shape[0] - 1)].numpy().reshape([4] * (2 * len(g.targets))),
+                     g.targets) for g in prog]
+            ref = fuse(host, dims, k)
+            assert len(ref) == len(blocks)
+            for (m, q), f in zip(blocks, ref):
+                assert q == f.qudits
+                got = m[min(b, m.shape[0] - 1)].numpy()
+                np.testing.assert_allclose(got, f.ptm.reshape(got.shape), atol=1e-14)
+            x0 = rng.standard_normal([4] * 5)
+            bl = [ob.general(2)] * 5
+            seq = _apply_all(bl, host, x0)
+            fused = _apply_all(bl, [(m[min(b, m.shape[0] - 1)].numpy().reshape([4] * (2 * len(q))),
+                                     q) for m, q in blocks], x0)
+            np.testing.assert_allclose(seq, fused, atol=1e-12)
+
+
+def test_fuse_batched_gradient_reaches_sources():
+    import torch
+    from quantumsim_amd.fusion import fuse_batched
+
+    class G:
+        def __init__(self, ptm, targets):
+            self.ptm, self.targets = ptm, targets
+
+    rng = np.random.default_rng(8)
+    a = torch.tensor(rng.standard_normal((2, 16, 16)), requires_grad=True)
+    b = torch.tensor(rng.standard_normal((2, 16, 16)), requires_grad=True)
+    blocks = fuse_batched([G(a, (0, 1)), G(b, (1, 2))], [4, 4, 4], 3)
+    assert len(blocks) == 1 and blocks[0][1] == (0, 1, 2)
+
+    def f(a_, b_):
+        m, _ = fuse_batched([G(a_, (0, 1)), G(b_, (1, 2))], [4, 4, 4], 3)[0]
+        return (m ** 2).sum()
+    assert torch.autograd.gradcheck(f, (a, b), eps=1e-6, atol=1e-6)

@@ -270,3 +270,50 @@ def test_emulated_grad_block_counts(blocks, rng):
     ref, _ = emulate_grad(nat.QSV_F64, x, y, targets, [4, 4])
     got, _ = emulate_grad(nat.QSV_F64, x, y, targets, [4, 4], grad_blocks=blocks)
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------ dense 64 x 64 batched paths
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets', [([4] * 5, (1, 2, 3)), ([4] * 5, (2, 3, 4)),
+                                          ([4] * 5, (0, 1, 2)), ([4] * 6, (4, 1, 3)),
+                                          ([4] * 3, (0, 1, 2)), ([4] * 4, (3, 2, 1)),
+                                          ([2] * 8, (1, 3, 4, 6, 7, 0))])
+def test_emulated_batched_dense64_matches_oracle(dims, targets, dtype, rng):
+    """The batched path's MFMA kernel for dense 64 x 64 PTMs (row-major PTM
+    gathered into fragment order per state), replayed on the CPU."""
+    pv = rng.standard_normal(dims)
+    din = [dims[t] for t in targets]
+    ptm = rng.standard_normal(din + din)
+    ref = oracle_apply(pv, ptm, targets)
+    out, path = emulate_tiled(dtype, pv, ptm, targets, nat.Tile(0, 0, 0, 0))
+    assert path & 7 == 4, path
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-12), (dims, targets, path)
+    # a launch shape is refused for dense PTMs
+    with pytest.raises(ValueError, match='no launch shape'):
+        emulate_tiled(dtype, pv, ptm, targets, nat.Tile(2, 64, 0, 0))
+
+
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets,dout', [
+    ([4] * 5, (1, 2, 3), [4, 4, 4]), ([4] * 5, (2, 3, 4), [4, 4, 4]),
+    ([4] * 5, (4, 0, 2), [4, 4, 4]), ([4] * 4, (0, 1, 2), [4, 2, 4]),
+    ([9, 4, 4, 9], (0, 1), [9, 4]), ([9, 4, 4, 9], (3, 2), [9, 2]),
+    ([4] * 3, (0, 1, 2), [4, 4, 4])])
+def test_emulated_grad64_matches_oracle(dims, targets, dout, dtype, rng):
+    """The wide (17..64) PTM gradient kernel, replayed: loads in bounds and
+    aligned, every orbit once, the MFMA tile maps of both element types."""
+    n = len(dims)
+    oshape = list(dims)
+    for t, d in zip(targets, dout):
+        oshape[t] = d
+    x = rng.standard_normal(dims)
+    ybar = rng.standard_normal(oshape)
+    xl = list(range(n))
+    yl = list(range(n))
+    outl = [n + j for j in range(len(targets))] + [t for t in targets]
+    for j, t in enumerate(targets):
+        yl[t] = n + j
+    ref = np.einsum(ybar, yl, x, xl, outl)
+    for blocks in (0, 1, 3, 64):
+        got, vn = emulate_grad(dtype, x, ybar, targets, dout, blocks)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (dims, targets, dout, blocks, vn)
