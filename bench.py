@@ -50,6 +50,9 @@ def parse_args():
                     help='skip the batched and differentiable measurements')
     ap.add_argument('--circuits', type=int, default=512)
     ap.add_argument('--param-sets', type=int, default=64)
+    ap.add_argument('--fuse-qudits', type=int, default=3,
+                    help='dense block size of the batched and differentiable paths '
+                         '(3: 64 x 64 MFMA blocks; 0: gate by gate)')
     return ap.parse_args()
 
 
@@ -121,7 +124,8 @@ def bench_batched(args):
     tuner = Autotuner(path=None)                 # tuned in this run, not read from disk
     set_tuner(tuner)
     t0 = time.perf_counter()
-    cb = CircuitBatch(circuits, dtype=torch.float32, checkpoint_every=16)
+    cb = CircuitBatch(circuits, dtype=torch.float32, checkpoint_every=16,
+                      fuse_qudits=args.fuse_qudits)
     w = cb.random_weights(7)
     cb.capture(w, autotune=True)
     t_setup = time.perf_counter() - t0
@@ -135,27 +139,35 @@ def bench_batched(args):
     for i, c in enumerate(circuits):
         by_n.setdefault(c['n'], []).append(i)
     sample = [by_n[n][j] for n in sorted(by_n) for j in range(min(2, len(by_n[n])))]
-    weights = np.random.default_rng(7)
-    per = []
-    for i in sample:
-        c = circuits[i]
-        wt = torch.tensor(weights.standard_normal((1, 2 ** c['n'])), dtype=torch.float64,
-                          device='cuda')
-        plain_forward_backward(c, wt)                  # warm-up of this shape
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        plain_forward_backward(c, wt)
-        torch.cuda.synchronize()
-        per.append(time.perf_counter() - t1)
-    # weight the per-width averages by the widths' share of the batch
-    mean_by_n = {}
-    for i, t in zip(sample, per):
-        mean_by_n.setdefault(circuits[i]['n'], []).append(t)
-    plain_sec = sum(len(by_n[n]) * statistics.mean(v) for n, v in mean_by_n.items())
+
+    def plain_seconds(fq):
+        weights = np.random.default_rng(7)
+        per = []
+        for i in sample:
+            c = circuits[i]
+            wt = torch.tensor(weights.standard_normal((1, 2 ** c['n'])), dtype=torch.float64,
+                              device='cuda')
+            plain_forward_backward(c, wt, fuse_qudits=fq)  # warm-up of this shape
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            plain_forward_backward(c, wt, fuse_qudits=fq)
+            torch.cuda.synchronize()
+            per.append(time.perf_counter() - t1)
+        # weight the per-width averages by the widths' share of the batch
+        mean_by_n = {}
+        for i, t in zip(sample, per):
+            mean_by_n.setdefault(circuits[i]['n'], []).append(t)
+        return sum(len(by_n[n]) * statistics.mean(v) for n, v in mean_by_n.items())
+    # the plain loop is timed gate by gate and with the same fusion; the
+    # faster of the two is the baseline
+    plain_by_fuse = {fq: plain_seconds(fq) for fq in sorted({0, args.fuse_qudits})}
+    plain_fq = min(plain_by_fuse, key=plain_by_fuse.get)
+    plain_sec = plain_by_fuse[plain_fq]
     # probability error of the float32 step against the same batch in float64
     cb.replay()
     torch.cuda.synchronize()
-    cb64 = CircuitBatch(circuits, dtype=torch.float64, checkpoint_every=16)
+    cb64 = CircuitBatch(circuits, dtype=torch.float64, checkpoint_every=16,
+                        fuse_qudits=args.fuse_qudits)
     with torch.no_grad():
         p64 = cb64.populations()
     prob_err = max((a - b).abs().max().item() for a, b in zip(cb.g_pops, p64))
@@ -164,9 +176,12 @@ def bench_batched(args):
            'circuits_per_s': round(len(circuits) / sec, 1),
            'plain_circuits_per_s': round(len(circuits) / plain_sec, 2),
            'speedup_vs_plain': round(plain_sec / sec, 2),
-           'plain_sample': len(sample), 'dtype': 'f32 state, f64 readout',
+           'plain_sample': len(sample), 'plain_fuse_qudits': plain_fq,
+           'plain_circuits_per_s_by_fuse': {str(k): round(len(circuits) / v, 2)
+                                            for k, v in plain_by_fuse.items()},
+           'dtype': 'f32 state, f64 readout',
            'checkpoint_every': 16, 'setup_s': round(t_setup, 1),
-           'max_prob_err_vs_f64': float('%.3e' % prob_err),
+           'max_prob_err_vs_f64': float('%.3e' % prob_err), 'fuse_qudits': args.fuse_qudits,
            'autotune': tuner.summary()}
     log('batched: %s' % res)
     del cb
@@ -226,16 +241,22 @@ def bench_differentiable(args):
     params = [ang.requires_grad_(), t1.requires_grad_(), t2.requires_grad_()]
 
     def step():
-        p = tc.populations(*params, torch.float32, checkpoint_every=8)
+        p = tc.populations(*params, torch.float32, checkpoint_every=8,
+                           fuse_qudits=args.fuse_qudits)
         loss = tc.kl(target, p).sum()
         return torch.autograd.grad(loss, params)
 
     step()
     torch.cuda.synchronize()
     sec = timed(step, 2, torch.cuda.current_stream())
+    with torch.no_grad():
+        first = [p.detach()[:1] for p in params]
+        n_gates = len(tc.gates(*first))
+        n_apps = len(tc.gates(*first, fuse_qudits=args.fuse_qudits))
     res = {'qubits': 12, 'layers': 40, 'params_per_set': tc.n_params, 'param_sets': sets,
            'fwd_adjoint_s': round(sec, 4), 'param_sets_per_s': round(sets / sec, 2),
-           'gates': len(tc.gates(*[p.detach()[:1] for p in params])), 'checkpoint_every': 8}
+           'gates': n_gates, 'applications': n_apps, 'fuse_qudits': args.fuse_qudits,
+           'checkpoint_every': 8}
     log('differentiable: %s' % res)
     return res
 

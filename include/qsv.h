@@ -138,7 +138,9 @@ int qsv_apply_ptm_dev_strided(int dtype, const void* in, void* out, int rank,
                               const int32_t* dims_out_t, const void* ptm_dev,
                               void* workspace, int64_t workspace_bytes, void* stream);
 
-/* Batched form of qsv_apply_ptm_dev for PTMs up to 16 x 16: `batch` states
+/* Batched form of qsv_apply_ptm_dev for PTMs up to 16 x 16 and dense
+ * 64 x 64 PTMs (three-qubit blocks, MFMA kernel, each state's row-major PTM
+ * gathered into fragment order inside the kernel): `batch` states
  * of identical shape, stored back to back (in: batch x N_in, out: batch x
  * N_out), each contracted with its own device PTM
  * ptm_dev + b * ptm_batch_stride (elements of the state's type; a stride of
@@ -191,12 +193,13 @@ int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64_t* dims, i
                                      const int32_t* targets, const int32_t* dims_out_t);
 
 /* Gradient of a PTM application with respect to the PTM, for `batch` states
- * back to back (the layout of qsv_apply_ptm_batched), PTMs up to 16 x 16:
+ * back to back (the layout of qsv_apply_ptm_batched), PTMs up to 64 x 64:
  *   dptm[b, o, i] (+)= sum over orbits of ybar[b, .., o, ..] * x[b, .., i, ..]
  * x: the states the PTM was applied to; ybar: the loss gradient with
  * respect to the results.  Accumulated in float64 in a fixed order
- * (deterministic), written (accumulate = 0) or added (accumulate = 1) to
- * dptm in the state's element type.  The gradient with respect to x is
+ * (deterministic; PTMs above 16 x 16 on float32 states sum in float32
+ * within a wave, float64 beyond), written (accumulate = 0) or added
+ * (accumulate = 1) to dptm in the state's element type.  The gradient with respect to x is
  * qsv_apply_ptm_batched with the transposed PTMs.  No counterpart in the
  * reference (which has no differentiation); it is the adjoint used by the
  * differentiable path. */
@@ -205,9 +208,9 @@ int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batch, int rank
                  void* dptm, int accumulate, void* workspace, int64_t workspace_bytes,
                  void* stream);
 
-/* qsv_ptm_grad with an explicit launch shape (tile->grad_blocks; NULL =
- * automatic).  The result does not depend on the shape beyond the order of
- * the float64 partial sums. */
+/* qsv_ptm_grad with an explicit launch shape (tile->grad_blocks, at most 64
+ * for PTMs above 16 x 16; NULL = automatic).  The result does not depend on
+ * the shape beyond the order of the partial sums. */
 int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, int batch, int rank,
                        const int64_t* dims, int k, const int32_t* targets,
                        const int32_t* dims_out_t, void* dptm, int accumulate, void* workspace,

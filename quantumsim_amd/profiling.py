@@ -12,6 +12,10 @@ plain copy, then runs the kernels under study.
   diagonal: the float64 population readout (2^n-entry gather) of the state
   batched : the 512-circuit batch tuned and captured, then a marker kernel
             (k_scale of 4,099 elements) and graph replays of forward + adjoint
+  differentiable: the 12-qubit, 40-layer, 96-parameter circuit over 64
+            parameter sets (float32, 3-qubit blocks, checkpoints every 8
+            applications): one warm-up step, the marker, then forward +
+            adjoint steps
 """
 import argparse
 import os
@@ -25,7 +29,8 @@ MARKER_ELEMS = 4099
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument('what', choices=['apply2q', 'apply1q', 'fused', 'batched', 'diagonal'])
+    ap.add_argument('what', choices=['apply2q', 'apply1q', 'fused', 'batched', 'diagonal',
+                                     'differentiable'])
     ap.add_argument('--qubits', type=int, default=14)
     ap.add_argument('--dtype', default='float64')
     ap.add_argument('--reps', type=int, default=3)
@@ -46,6 +51,29 @@ def main(argv=None):
     del src, dst
     print('copy reference: %d bytes read, %d bytes written' % (N * src_bytes(dt), N * src_bytes(dt)),
           flush=True)
+    if a.what == 'differentiable':
+        from quantumsim_amd.workloads import TrainableCircuit
+        from quantumsim_amd import _native as nat
+        tc = TrainableCircuit(n=12, layers=40)
+        ang, t1, t2 = tc.initial_params(64, seed=11, device='cuda', dtype=torch.float64)
+        with torch.no_grad():
+            target = tc.populations(ang + 0.1, t1 * 1.05, t2 * 0.97, torch.float32)
+        params = [ang.requires_grad_(), t1.requires_grad_(), t2.requires_grad_()]
+
+        def step():
+            p = tc.populations(*params, torch.float32, checkpoint_every=8, fuse_qudits=3)
+            return torch.autograd.grad(tc.kl(target, p).sum(), params)
+        step()
+        torch.cuda.synchronize()
+        m = torch.ones(MARKER_ELEMS, dtype=torch.float64, device='cuda')
+        nat.check(nat.lib().qsv_scale(nat.QSV_F64, m.data_ptr(), m.numel(), 1.0,
+                                      torch.cuda.current_stream().cuda_stream), 'scale')
+        torch.cuda.synchronize()
+        for _ in range(a.reps):
+            step()
+        torch.cuda.synchronize()
+        print('done', flush=True)
+        return 0
     if a.what == 'batched':
         from quantumsim_amd.batch import CircuitBatch
         from quantumsim_amd.workloads import random_circuit_batch

@@ -258,7 +258,7 @@ def differentiable(sets):
         target = tc.populations(ang + 0.1, t1 * 1.05, t2 * 0.97, torch.float64)
 
     def loss(params, dtype):
-        p = tc.populations(*params, dtype, checkpoint_every=8)
+        p = tc.populations(*params, dtype, checkpoint_every=8, fuse_qudits=3)
         return tc.kl(target, p).sum()
 
     params = [ang.clone().requires_grad_(), t1.clone().requires_grad_(),
