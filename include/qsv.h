@@ -223,7 +223,8 @@ int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, int batch, in
  * exactly once.  x, ybar: host float64 arrays (contiguous); dptm: host
  * float64 [dout * din].  grad_blocks: blocks per state, as the qsv_tile field (0:
  * automatic for a batch of one).  *vn_out (may be NULL) = orbits per lane
- * vector the GPU call would use for `dtype`.  No counterpart in the reference; it lets
+ * vector the GPU call would use for `dtype`, plus 256 when 64 x 64 PTMs
+ * load runs of four entries as 16-byte vectors (innermost axis a target).  No counterpart in the reference; it lets
  * CPU tests pin the gradient kernel's indexing before GPU use. */
 int qsv_emulate_ptm_grad(int dtype, const double* x, const double* ybar, int rank,
                          const int64_t* dims, int k, const int32_t* targets,

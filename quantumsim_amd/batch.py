@@ -242,13 +242,16 @@ class CircuitBatch:
             while layer < g.layers[0] and deeper(layer) == active:
                 gates += self._window_gates(g, ptms, layer, active)
                 layer += self.window
-            x = run_circuit(x[:active], gates, (4,) * n, self.k)
+            x = run_circuit(x, gates, (4,) * n, self.k)
             nxt = deeper(layer)
+            # one split into the continuing prefix and the finished states:
+            # its backward is one concatenation (separate slices would each
+            # backpropagate a zero-filled full-size gradient to be summed)
+            parts = torch.split(x, [nxt] + [1] * (active - nxt))
             for b in range(nxt, active):
-                finals[b] = x[b:b + 1]
+                finals[b] = parts[1 + b - nxt]
             active = nxt
-            if active:
-                x = x[:active]
+            x = parts[0]
         states = torch.cat(finals, 0)
         return diagonal(states, [general(2)] * n)
 

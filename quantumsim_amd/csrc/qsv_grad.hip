@@ -110,12 +110,19 @@ __global__ void __launch_bounds__(kBlock)
 constexpr int kGrad64Blocks = 64;
 constexpr int kGrad64Unroll = 1;
 
-template <typename T, int VN>
+// QUAD (64 x 64 PTMs whose lowest index has unit stride, i.e. the
+// innermost axis is a target): lane c holds the four consecutive entries
+// 4c .. 4c+3 of its orbit, loaded as 16-byte vectors, and row (column) block
+// q of the tiles is the entries {4 m + q}; the contraction does not depend
+// on which entry sits in which MFMA row, only the epilogue's map does.
+template <typename T, int VN, bool QUAD>
 __global__ void __launch_bounds__(kBlock)
     k_ptm_grad64(const T* __restrict__ x, const T* __restrict__ ybar, const Geom g,
                  const GradOffs offs, const uint32_t n_orbits, double* __restrict__ partial) {
   typedef typename Mfma<T>::acc Acc;
   typedef typename VT<T, VN>::t Vec;
+  constexpr int QV = 16 / sizeof(T);  // elements per 16-byte vector
+  typedef typename VT<T, QV>::t QVec;
   __shared__ double red[64 * 64];
   const uint32_t b = blockIdx.y;
   x += int64_t(b) * g.bs_in;
@@ -126,10 +133,11 @@ __global__ void __launch_bounds__(kBlock)
   bool vi[4], vo[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    vi[q] = 16 * q + c < g.din;
-    vo[q] = 16 * q + c < g.dout;
-    oi[q] = vi[q] ? offs.off_in[16 * q + c] : 0;
-    oo[q] = vo[q] ? offs.off_out[16 * q + c] : 0;
+    const int ent = QUAD ? 4 * c : 16 * q + c;
+    vi[q] = ent < g.din;
+    vo[q] = ent < g.dout;
+    oi[q] = vi[q] ? offs.off_in[ent] : 0;
+    oo[q] = vo[q] ? offs.off_out[ent] : 0;
   }
   Acc acc[4][4];
 #pragma unroll
@@ -149,13 +157,29 @@ __global__ void __launch_bounds__(kBlock)
       const bool ok = m < n_orbits;  // VN divides n_orbits: whole vectors
       int64_t bin, bout;
       orbit_base(g, ok ? m : 0, bin, bout);
+      if constexpr (QUAD) {
+        // entries 4c .. 4c+3 (all valid: QUAD needs a 64 x 64 PTM)
+        const T* px = x + (ok ? bin + oi[0] : 0);
+        const T* py = ybar + (ok ? bout + oo[0] : 0);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const bool lx = ok && vi[q], ly = ok && vo[q];
-        xv[u][q] = *reinterpret_cast<const Vec*>(x + (lx ? bin + oi[q] : 0));
-        yv[u][q] = *reinterpret_cast<const Vec*>(ybar + (ly ? bout + oo[q] : 0));
-        if (!lx) xv[u][q] = Vec(0);
-        if (!ly) yv[u][q] = Vec(0);
+        for (int h = 0; h < 4 / QV; ++h) {
+          const QVec a = *reinterpret_cast<const QVec*>(px + h * QV);
+          const QVec bb = *reinterpret_cast<const QVec*>(py + h * QV);
+#pragma unroll
+          for (int e = 0; e < QV; ++e) {
+            xv[u][h * QV + e] = ok ? a[e] : T(0);
+            yv[u][h * QV + e] = ok ? bb[e] : T(0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const bool lx = ok && vi[q], ly = ok && vo[q];
+          xv[u][q] = *reinterpret_cast<const Vec*>(x + (lx ? bin + oi[q] : 0));
+          yv[u][q] = *reinterpret_cast<const Vec*>(ybar + (ly ? bout + oo[q] : 0));
+          if (!lx) xv[u][q] = Vec(0);
+          if (!ly) yv[u][q] = Vec(0);
+        }
       }
     }
 #pragma unroll
@@ -172,8 +196,9 @@ __global__ void __launch_bounds__(kBlock)
               acc[rb][cb] = Mfma<T>::op(yv[u][rb][e], xv[u][cb][e], acc[rb][cb]);
           }
   }
-  // slot r of lane (c, k) in tile (rb, cb) holds entry (16 rb + row, 16 cb + c)
-  // with row = k + 4 r (float64 MFMA) or 4 k + r (float32 MFMA)
+  // slot r of lane (c, k) in tile (rb, cb) holds MFMA row `row` = k + 4 r
+  // (float64) or 4 k + r (float32) and column c, i.e. entry
+  // (16 rb + row, 16 cb + c), or (4 row + rb, 4 c + cb) with QUAD
 #pragma unroll
   for (int rb = 0; rb < 4; ++rb) {
 #pragma unroll
@@ -184,7 +209,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int row = sizeof(T) == 8 ? k + 4 * r : 4 * k + r;
-            const int idx = (16 * rb + row) * 64 + 16 * cb + c;
+            const int idx = QUAD ? (4 * row + rb) * 64 + 4 * c + cb
+                                 : (16 * rb + row) * 64 + 16 * cb + c;
             red[idx] = (ww == 0 ? 0.0 : red[idx]) + double(acc[rb][cb][r]);
           }
       }
@@ -219,7 +245,8 @@ struct GradPlan {
   GradOffs offs;
   uint32_t n_orbits;
   int din, dout;
-  int vn;  // orbits per 16-byte lane vector (1: scalar loads)
+  int vn;    // orbits per 16-byte lane vector (1: scalar loads)
+  int quad;  // 64 x 64 with a unit-stride lowest index: 16-byte loads of 4 entries
   int64_t n_in, n_out;
 };
 
@@ -325,6 +352,19 @@ static int make_grad_plan(int elem_bytes, int rank, const int64_t* dims, int k,
     for (int o = 0; o < p.dout; ++o) aligned = aligned && p.offs.off_out[o] % want == 0;
     if (aligned) p.vn = want;
   }
+  // 64 x 64 PTMs whose entries come in aligned runs of four (the innermost
+  // axis is a target of Pauli dimension 4)
+  p.quad = 0;
+  if (p.vn == 1 && p.din == 64 && p.dout == 64 && p.n_in % want == 0 && p.n_out % want == 0) {
+    bool ok = true;
+    for (int e = 0; e < 64; ++e)
+      ok = ok && p.offs.off_in[e] == p.offs.off_in[e & ~3] + (e & 3) &&
+           p.offs.off_out[e] == p.offs.off_out[e & ~3] + (e & 3);
+    for (int e = 0; e < 64; e += 4)
+      ok = ok && p.offs.off_in[e] % want == 0 && p.offs.off_out[e] % want == 0;
+    for (int j = 0; j < ng; ++j) ok = ok && g.gs_in[j] % want == 0 && g.gs_out[j] % want == 0;
+    if (ok) p.quad = 1;
+  }
   return QSV_OK;
 }
 
@@ -366,17 +406,20 @@ extern "C" int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64
 }
 
 template <typename T>
-static void launch_grad(const GradPlan& p, int vn, dim3 grid, const void* x, const void* ybar,
-                        double* partial, hipStream_t s) {
+static void launch_grad(const GradPlan& p, int vn, bool quad, dim3 grid, const void* x,
+                        const void* ybar, double* partial, hipStream_t s) {
   const T* xx = static_cast<const T*>(x);
   const T* yy = static_cast<const T*>(ybar);
   if (grad_wide(p)) {
     if (vn == 16 / int(sizeof(T)))
-      hipLaunchKernelGGL((k_ptm_grad64<T, 16 / sizeof(T)>), grid, dim3(kBlock), 0, s, xx, yy, p.g,
+      hipLaunchKernelGGL((k_ptm_grad64<T, 16 / sizeof(T), false>), grid, dim3(kBlock), 0, s, xx,
+                         yy, p.g, p.offs, p.n_orbits, partial);
+    else if (quad)
+      hipLaunchKernelGGL((k_ptm_grad64<T, 1, true>), grid, dim3(kBlock), 0, s, xx, yy, p.g,
                          p.offs, p.n_orbits, partial);
     else
-      hipLaunchKernelGGL((k_ptm_grad64<T, 1>), grid, dim3(kBlock), 0, s, xx, yy, p.g, p.offs,
-                         p.n_orbits, partial);
+      hipLaunchKernelGGL((k_ptm_grad64<T, 1, false>), grid, dim3(kBlock), 0, s, xx, yy, p.g,
+                         p.offs, p.n_orbits, partial);
     return;
   }
   if (vn == 16 / int(sizeof(T)))
@@ -425,10 +468,11 @@ extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, in
   const bool aligned = reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
                        reinterpret_cast<uintptr_t>(ybar) % 16 == 0;
   const int vn = aligned ? p.vn : 1;
+  const bool quad = aligned && p.quad;
   if (dtype == QSV_F64)
-    launch_grad<double>(p, vn, grid, x, ybar, partial, s);
+    launch_grad<double>(p, vn, quad, grid, x, ybar, partial, s);
   else
-    launch_grad<float>(p, vn, grid, x, ybar, partial, s);
+    launch_grad<float>(p, vn, quad, grid, x, ybar, partial, s);
   QSV_CHECK_HIP(hipGetLastError());
   const int dd = p.din * p.dout;
   const dim3 g2((dd + kBlock - 1) / kBlock, batch);
@@ -483,6 +527,20 @@ static int emulate_grad64(const GradPlan& p, int es, const double* x, const doub
                 double& bb = B[(size_t(q) * 64 + lane) * 16 + e];
                 a = bb = 0.0;
                 if (!ok) continue;
+                if (p.quad) {
+                  // entries 4c + q: one run of four from the (aligned) first
+                  const int QV = 16 / es;
+                  const int64_t ai = bin + p.offs.off_in[4 * c], ao = bout + p.offs.off_out[4 * c];
+                  QSV_CHECK_ARG(ai % QV == 0 && ao % QV == 0, "grad64: misaligned quad load");
+                  QSV_CHECK_ARG(p.offs.off_in[4 * c + q] == p.offs.off_in[4 * c] + q &&
+                                    p.offs.off_out[4 * c + q] == p.offs.off_out[4 * c] + q,
+                                "grad64: quad entries not contiguous");
+                  QSV_CHECK_ARG(ai + 3 < p.n_in && ao + 3 < p.n_out && ai >= 0 && ao >= 0,
+                                "grad64: quad load out of bounds");
+                  bb = x[ai + q];
+                  a = ybar[ao + q];
+                  continue;
+                }
                 const int ent = 16 * q + c;
                 if (ent < p.din) {
                   const int64_t ad = bin + p.offs.off_in[ent] + e;
@@ -524,7 +582,8 @@ static int emulate_grad64(const GradPlan& p, int es, const double* x, const doub
           for (int lane = 0; lane < 64; ++lane)
             for (int r = 0; r < 4; ++r) {
               const int row = es == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
-              const int idx = (16 * rb + row) * 64 + 16 * cb + (lane & 15);
+              const int idx = p.quad ? (4 * row + rb) * 64 + 4 * (lane & 15) + cb
+                                     : (16 * rb + row) * 64 + 16 * cb + (lane & 15);
               const double v = acc[((((size_t(w) * 4 + rb) * 4 + cb) * 64) + lane) * 4 + r];
               red[idx] = (w == 0 ? 0.0 : red[idx]) + v;
             }
@@ -561,7 +620,7 @@ extern "C" int qsv_emulate_ptm_grad(int dtype, const double* x, const double* yb
                 "emulation is for small vectors");
   QSV_CHECK_ARG(grad_blocks_forced >= 0 && grad_blocks_forced <= max_grad_blocks(p),
                 "grad_blocks outside [0, %d]", max_grad_blocks(p));
-  if (vn_out) *vn_out = p.vn;
+  if (vn_out) *vn_out = p.vn | (p.quad << 8);
   if (grad_wide(p))
     return emulate_grad64(p, dtype_size(dtype), x, ybar,
                           grad_blocks_forced > 0 ? grad_blocks_forced

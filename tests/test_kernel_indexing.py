@@ -317,3 +317,16 @@ def test_emulated_grad64_matches_oracle(dims, targets, dout, dtype, rng):
     for blocks in (0, 1, 3, 64):
         got, vn = emulate_grad(dtype, x, ybar, targets, dout, blocks)
         assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (dims, targets, dout, blocks, vn)
+
+
+def test_grad64_load_modes():
+    """Wide gradient: orbit vectors when the innermost axis is a bystander,
+    runs of four entries (16-byte loads along the PTM index) when it is a
+    target, scalar loads otherwise."""
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal([4] * 5)
+    for dtype, vec in ((nat.QSV_F64, 2), (nat.QSV_F32, 4)):
+        assert emulate_grad(dtype, x, x, (1, 2, 3), [4, 4, 4])[1] == vec
+        assert emulate_grad(dtype, x, x, (2, 3, 4), [4, 4, 4])[1] == 1 | 256
+        assert emulate_grad(dtype, x, x, (4, 2, 3), [4, 4, 4])[1] == 1
+        assert emulate_grad(dtype, x, x, (0, 1, 4), [4, 4, 4])[1] == 1 | 256
