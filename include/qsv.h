@@ -125,9 +125,10 @@ int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
                       void* workspace, int64_t workspace_bytes, void* stream);
 
 /* qsv_apply_ptm_dev on strided tensors (strides as in qsv_apply_ptm_strided).
- * Dense 64 x 64 / 256 x 256 PTMs are packed into the MFMA kernels' fragment
- * order in the workspace by a device kernel on `stream`, so a PTM built on
- * the device never travels to the host.  This is the backend call of the
+ * Dense 64 x 64 PTMs are gathered into the MFMA kernel's fragment order by
+ * the kernel itself, 256 x 256 ones packed in the workspace by a device
+ * kernel on `stream`, so a PTM built on the device never travels to the
+ * host.  This is the backend call of the
  * device-side operation pipeline: the PTM of quantumsim's
  * PTMOperation.__call__ (quantumsim/operations/operation.py:529-551),
  * built and basis-converted with torch, applied as

@@ -1070,8 +1070,11 @@ extern "C" int qsv_apply_ptm_dev_strided(int dtype, const void* in, void* out, i
   }
   FusedPlan fp;
   if (make_fused_plan(p, dtype_size(dtype), in, out, fp)) {
-    // dense 64 x 64 / 256 x 256: the PTM is packed into MFMA fragment order
-    // in the workspace by a kernel on the same stream (no host copy)
+    // dense 64 x 64: the kernel gathers the row-major PTM into fragment
+    // order itself; 256 x 256: packed into fragment order in the workspace
+    // by a kernel on the same stream.  No host copy either way.
+    if (fp.D == 64)
+      return launch_fused64_batched(dtype, fp, p.g, p.nt, in, out, ptm_dev, 0, 1, s);
     const int64_t need = int64_t(fp.D) * fp.D * dtype_size(dtype);
     QSV_CHECK_ARG(workspace && workspace_bytes >= need,
                   "a %d x %d PTM needs a workspace of %lld bytes", fp.D, fp.D, (long long)need);
