@@ -264,7 +264,7 @@ def bench_differentiable(args):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build has no multi-GPU path '
+        print('bench.py measures one GPU: there is no multi-GPU path '
               '(--gpus %d requested)' % args.gpus, file=sys.stderr)
         sys.exit(2)
     import numpy as np

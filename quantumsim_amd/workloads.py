@@ -190,7 +190,7 @@ class TrainableCircuit:
     cyclically (block (layer // 5) mod 3: 3 x 12 x 2 = 72 angles), plus T1
     and T2 of every qubit (24).  (Eight independent blocks plus T1/T2 would
     be 216 parameters, not the 96 of the benchmark; the cyclic three-block
-    split is this build's choice.)  The loss is KL(target || simulated) over
+    split is the choice made here.)  The loss is KL(target || simulated) over
     the 2^n populations; the target comes from the same circuit with
     seeded, perturbed parameters."""
 
