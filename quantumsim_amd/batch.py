@@ -194,41 +194,59 @@ class CircuitBatch:
             self._consts[key] = t
         return t
 
-    def _window_gates(self, g, ptms, l, active):
-        """Gates of layers [l, l + window) for the `active` deepest circuits
-        of group g: a circuit that ends inside the window gets identity PTMs
-        for its missing layers (its state is final after the window), and
-        with fusion the window's gates are composed into dense blocks of up
-        to ``fuse_qudits`` qubits (workloads.fused_layer_gates -> at most
-        n/2 + 1 two-qubit PTMs per layer -> about n/3 three-qubit blocks per
-        two layers)."""
-        n = g.n
+    def _window_blocks(self, g):
+        """Dense blocks of every two-layer window of group g, composed for
+        all windows and circuits at once (the windows share one fusion plan):
+        a list over block positions of ([n_windows, B, D, D] in the state's
+        element type, block qubits), so window w's PTMs for the running
+        prefix are a contiguous slice.  Layer 2w + 1 of a
+        circuit with at most 2w + 1 layers (and the padding layer of an odd
+        maximum depth) is the identity: its state is final after the
+        window."""
+        n, B = g.n, len(g.members)
+        Lmax = g.layers[0]
+        nw = (Lmax + 1) // 2
+        ang = g.angles
+        if 2 * nw > Lmax:                 # pad to whole windows (masked below)
+            ang = torch.cat([ang, torch.zeros_like(ang[:, :1])], 1)
+        rot = rotation_ptms(ang)                                  # [B, 2 nw, n, 4, 4]
+        damp = damping_ptms(g.t1, g.t2)                           # [B, n, 4, 4]
+        damp_w = damp[:, None].expand(B, nw, n, 4, 4).reshape(B * nw, n, 4, 4)
+        keep = self._cached(('keep', id(g)), lambda: torch.tensor(
+            [g.layers[b] > 2 * w + 1 for b in range(B) for w in range(nw)],
+            dtype=torch.bool, device=self.device)[:, None, None])
         gates = []
-        for j in range(self.window):
-            lj = l + j
-            if lj >= g.layers[0]:
-                break
-            keep = [g.layers[b] > lj for b in range(active)]
-            mask = None
-            if not all(keep):
-                mask = self._cached(('mask', id(g), lj, active), lambda: torch.tensor(
-                    keep, dtype=torch.bool, device=self.device)[:, None, None])
-            for P, t in ptms[lj % 2]:
-                pt = P[lj // 2][:active]
-                if mask is not None:
-                    D = pt.shape[-1]
-                    eye = self._cached(('eye', D, pt.dtype), lambda: torch.eye(
-                        D, dtype=pt.dtype, device=self.device))
-                    pt = torch.where(mask, pt, eye)
-                gates.append(Gate(pt, t))
-        if self.fuse_qudits > 2 and len(gates) > 1:
-            gates = [Gate(m, q) for m, q in fuse_batched(gates, [4] * n, self.fuse_qudits)]
-        return gates
+        for par in (0, 1):
+            rot_p = rot[:, par::2].reshape(B * nw, n, 4, 4)
+            for gt in fused_layer_gates(n, par, rot_p, damp_w):
+                ptm = gt.ptm
+                if par == 1:
+                    D = ptm.shape[-1]
+                    eye = self._cached(('eye', D, ptm.dtype), lambda: torch.eye(
+                        D, dtype=ptm.dtype, device=self.device))
+                    ptm = torch.where(keep, ptm, eye)
+                gates.append(Gate(ptm, gt.targets))
+        out = []
+        for m, q in fuse_batched(gates, [4] * n, self.fuse_qudits):
+            D = m.shape[-1]
+            out.append((m.reshape(B, nw, D, D).transpose(0, 1).to(self.dtype).contiguous(), q))
+        return out
+
+    def _window_gates(self, g, ptms, l, active):
+        """Gates of layer l for the `active` deepest circuits of group g
+        (one layer per window: no fusion)."""
+        return [Gate(P[l // 2][:active], t) for P, t in ptms[l % 2]]
 
     def _group_populations(self, g):
         n, B = g.n, len(g.members)
         N = 4 ** n
-        ptms = self._layer_ptms(g)
+        if self.window == 2:
+            # one view per window (unbind: its backward is a single stack;
+            # slices of the stacked tensor would each backpropagate a
+            # zero-filled copy of all windows)
+            blocks = [(m.unbind(0), q) for m, q in self._window_blocks(g)]
+        else:
+            ptms = self._layer_ptms(g)
         x = torch.zeros((B, N), dtype=self.dtype, device=self.device)
         x[:, 0] = 1
         finals = [None] * B
@@ -240,7 +258,11 @@ class CircuitBatch:
             # window's first layer) is unchanged: one checkpointed segment
             gates = []
             while layer < g.layers[0] and deeper(layer) == active:
-                gates += self._window_gates(g, ptms, layer, active)
+                if self.window == 2:
+                    w = layer // 2
+                    gates += [Gate(m[w][:active], q) for m, q in blocks]
+                else:
+                    gates += self._window_gates(g, ptms, layer, active)
                 layer += self.window
             x = run_circuit(x, gates, (4,) * n, self.k)
             nxt = deeper(layer)
