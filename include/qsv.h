@@ -209,6 +209,22 @@ int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batch, int rank
                  void* dptm, int accumulate, void* workspace, int64_t workspace_bytes,
                  void* stream);
 
+/* Backward of qsv_apply_ptm_batched for a square dense 64 x 64 PTM in one
+ * pass: xbar = P^T ybar (per state, P = ptm_dev + b * ptm_batch_stride,
+ * row-major in the state's element type) and dptm (+)= sum over orbits of
+ * ybar x^T as qsv_ptm_grad computes it; reads x and ybar once and writes
+ * xbar (three streams instead of four).  workspace: qsv_ptm_grad_workspace_bytes
+ * of the same geometry.  QSV_ENOTSUP unless the innermost axis is a
+ * unit-stride bystander and the states are 16-byte aligned (callers then use
+ * qsv_apply_ptm_batched with the transposed PTM and qsv_ptm_grad).  xbar
+ * must not alias x or ybar.  No counterpart in the reference; the adjoint
+ * of the differentiable and batched paths. */
+int qsv_apply_ptm_batched_backward(int dtype, const void* x, const void* ybar, void* xbar,
+                                   int batch, int rank, const int64_t* dims, int k,
+                                   const int32_t* targets, const void* ptm_dev,
+                                   int64_t ptm_batch_stride, void* dptm, int accumulate,
+                                   void* workspace, int64_t workspace_bytes, void* stream);
+
 /* qsv_ptm_grad with an explicit launch shape (tile->grad_blocks, at most 64
  * for PTMs above 16 x 16; NULL = automatic).  The result does not depend on
  * the shape beyond the order of the partial sums. */
@@ -305,6 +321,15 @@ int qsv_emulate_circuits(int n_circuits, const qsv_circuit* circs, const qsv_gat
                          int64_t n_gates_total, const double* ptms, int64_t ptm_elems,
                          double* traj, int64_t traj_elems, double* pops, int64_t pop_elems,
                          const double* dpops, double* dptms, int max_qubits);
+
+/* CPU replay of qsv_apply_ptm_batched_backward for one state (host float64
+ * arrays; ptm row-major 64 x 64): every access checked, xbar written once
+ * per entry, the MFMA maps of both parts; grad_blocks as in
+ * qsv_emulate_ptm_grad.  No counterpart in the reference. */
+int qsv_emulate_apply_ptm_batched_backward(int dtype, const double* x, const double* ybar,
+                                           double* xbar, int rank, const int64_t* dims, int k,
+                                           const int32_t* targets, const double* ptm,
+                                           double* dptm, int32_t grad_blocks);
 
 /* qsv_emulate_apply_ptm of the batched path's kernel choice for one state,
  * with an explicit launch shape (the autotuner's candidates are replayed

@@ -71,6 +71,14 @@ _SIGNATURES = {
     'qsv_emulate_ptm_grad': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, _c_i64p,
                                             ctypes.c_int, _c_i32p, _c_i32p, _c_f64p,
                                             ctypes.c_int32, _c_i32p]),
+    'qsv_apply_ptm_batched_backward': (ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
+                                                      ctypes.c_int, _c_i64p, ctypes.c_int,
+                                                      _c_i32p, _vp, ctypes.c_int64, _vp,
+                                                      ctypes.c_int, _vp, ctypes.c_int64, _vp]),
+    'qsv_emulate_apply_ptm_batched_backward': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p,
+                                                              _c_f64p, ctypes.c_int, _c_i64p,
+                                                              ctypes.c_int, _c_i32p, _c_f64p,
+                                                              _c_f64p, ctypes.c_int32]),
     'qsv_circuits_max_qubits': (ctypes.c_int, [ctypes.c_int]),
     'qsv_circuits_lds_bytes': (ctypes.c_int64, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     'qsv_circuits_forward': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,

@@ -69,10 +69,10 @@ __host__ __device__ inline int64_t fused_frag_slot(int D, int s, int mb, int lan
   return (int64_t(s) * (D / 16) + mb) * 64 + lane;
 }
 
-// Row-major PTM index (o * D + k) whose value sits at fragment slot `slot`
-// (the inverse of fused_frag_slot composed with the row/column maps).
-__host__ __device__ inline int64_t fused_frag_source(int D, int elem_bytes, int vi, int vo,
-                                                     int64_t slot) {
+// PTM row o and column k whose value sits at fragment slot `slot` (the
+// inverse of fused_frag_slot composed with the row/column maps).
+__host__ __device__ inline void fused_frag_coords(int D, int elem_bytes, int vi, int vo,
+                                                  int64_t slot, int& o, int& k) {
   const int MB = D / 16;
   const int lane = int(slot & 63);
   const int64_t sm = slot >> 6;
@@ -81,7 +81,16 @@ __host__ __device__ inline int64_t fused_frag_source(int D, int elem_bytes, int 
   const int m = lane & 15, jk = lane >> 4;
   int j, r;
   mfma_row_owner(elem_bytes, m, j, r);
-  return int64_t(fused_o(vo, 4 * mb + r, j)) * D + fused_k(vi, s, jk);
+  o = fused_o(vo, 4 * mb + r, j);
+  k = fused_k(vi, s, jk);
+}
+
+// Row-major PTM index (o * D + k) of fragment slot `slot`.
+__host__ __device__ inline int64_t fused_frag_source(int D, int elem_bytes, int vi, int vo,
+                                                     int64_t slot) {
+  int o, k;
+  fused_frag_coords(D, elem_bytes, vi, vo, slot, o, k);
+  return int64_t(o) * D + k;
 }
 
 // Fills pf[D * D] (fragment order) from the row-major float64 PTM.

@@ -17,6 +17,7 @@
 // contiguous run when the targets are the innermost axes).  Blocks write
 // float64 partial tiles, a second kernel sums them in a fixed order
 // (deterministic).
+#include "qsv_fused.h"
 #include "qsv_geom.h"
 #include "qsv_mfma.h"
 
@@ -392,6 +393,193 @@ static int grad64_blocks(uint32_t n_orbits, int batch) {
 
 static int max_grad_blocks(const GradPlan& p) { return grad_wide(p) ? kGrad64Blocks : kGradBlocks; }
 
+// Fused backward of a dense 64 x 64 block (both element types) when the
+// innermost axis is a unit-stride bystander:
+//   xbar = P^T ybar   and   dP += sum over orbits ybar x^T
+// in one pass, reading x and ybar once from HBM and writing xbar (three
+// streams instead of the four of a transposed apply plus a gradient).  A
+// wave walks tiles of 16 VN orbits; per tile:
+//   A. the k_fused64 contraction with the transposed PTM (gathered into
+//      fragment order in LDS): lane (n, j) loads VN-orbit vectors of ybar
+//      for entries 4 s + j, the accumulators are stored to xbar;
+//   B. the k_ptm_grad64 contraction over the same orbits, four k-steps of
+//      4 VN orbits: lane (c, k) loads entries 16 q + c of x and of ybar
+//      (ybar's second read of the tile comes from the caches).
+// The dP tiles stay in registers across tiles; at the end the four waves
+// are summed in LDS (the PTM's space, after a barrier) in a fixed order and
+// the block writes its partial, summed by k_ptm_grad_final.
+template <typename T, int VN>
+__global__ void __launch_bounds__(kBlock)
+    k_bwd64(const T* __restrict__ x, const T* __restrict__ ybar, T* __restrict__ xbar,
+            const Geom g, const uint32_t n_orbits, const T* __restrict__ ptm,
+            const int64_t pstride, const GradOffs offs, const FusedOffs fo,
+            double* __restrict__ partial) {
+  constexpr int MB = 4, KS = 16, VH = 2;
+  constexpr uint32_t COLS = 16 * VN, STEP = 4 * VN;
+  typedef typename Mfma<T>::acc Acc;
+  typedef typename VT<T, VN>::t Vec;
+  typedef typename VT<T, VH>::t VHv;
+  __shared__ double smem[64 * 64];
+  T* As = reinterpret_cast<T*>(smem);
+  const uint32_t b = blockIdx.y;
+  x += int64_t(b) * g.bs_in;
+  xbar += int64_t(b) * g.bs_in;
+  ybar += int64_t(b) * g.bs_out;
+  ptm += int64_t(b) * pstride;
+  for (int i = threadIdx.x; i < KS * MB * 64; i += kBlock) {
+    int o, kk;
+    fused_frag_coords(64, int(sizeof(T)), 1, 1, i, o, kk);
+    As[i] = ptm[int64_t(kk) * 64 + o];  // fragment (o, kk) of P^T
+  }
+  __syncthreads();
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int n = lane & 15, j = lane >> 4;  // part A: orbit column n, k-group j
+  const int c = n, kq = j;                 // part B: entry c, orbit group kq
+  const uint32_t nw = gridDim.x * (kBlock / 64);
+  const uint32_t wid = blockIdx.x * (kBlock / 64) + w;
+  const uint32_t full = n_orbits / COLS;
+  const int64_t li = fo.in_l[j], lo = fo.out_l[j];
+  int64_t oi[4], oo[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    oi[q] = offs.off_in[16 * q + c];
+    oo[q] = offs.off_out[16 * q + c];
+  }
+  Acc dacc[4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) dacc[rb][cb] = Acc(0);
+
+  auto tile = [&](uint32_t t, bool tail) {
+    // A: xbar = P^T ybar on orbits t COLS + VN n + e, in halves of VH
+    // orbits (float32: 8-byte accesses, still whole 128-byte lines over 16
+    // lanes) to keep the registers of both parts within two waves per SIMD
+    {
+      const uint32_t m0 = t * COLS + VN * n;
+      const uint32_t m = m0 < n_orbits ? m0 : n_orbits - VN;  // tail: loaded, not stored
+      int64_t bin, bout;
+      orbit_base(g, m, bin, bout);
+      const T* py = ybar + bout + li;
+      T* px = xbar + bin + lo;
+      const bool st = !tail || m0 < n_orbits;
+#pragma unroll
+      for (int h = 0; h < VN / VH; ++h) {
+        T bv[VH][KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+          const VHv v = *reinterpret_cast<const VHv*>(py + h * VH + fo.in_u[s]);
+#pragma unroll
+          for (int e = 0; e < VH; ++e) bv[e][s] = v[e];
+        }
+        Acc acc[MB][VH];
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+          for (int nb = 0; nb < VH; ++nb) acc[mb][nb] = Acc(0);
+        int abase = lane;
+        asm volatile("" : "+v"(abase));
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+#pragma unroll
+          for (int mb = 0; mb < MB; ++mb) {
+            const T a = As[(s * MB + mb) * 64 + abase];
+#pragma unroll
+            for (int nb = 0; nb < VH; ++nb) acc[mb][nb] = Mfma<T>::op(a, bv[nb][s], acc[mb][nb]);
+          }
+        if (st) {
+#pragma unroll
+          for (int q = 0; q < 4 * MB; ++q) {
+            VHv v;
+#pragma unroll
+            for (int e = 0; e < VH; ++e) v[e] = acc[q >> 2][e][q & 3];
+            *reinterpret_cast<VHv*>(px + h * VH + fo.out_u[q]) = v;
+          }
+        }
+      }
+    }
+    // B: dP over the same orbits, four k-steps of 4 VN orbits
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      const uint32_t m = t * COLS + tt * STEP + VN * kq;
+      const bool ok = m < n_orbits;
+      int64_t bin, bout;
+      orbit_base(g, ok ? m : 0, bin, bout);
+      Vec xv[4], yv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        xv[q] = *reinterpret_cast<const Vec*>(x + (ok ? bin + oi[q] : 0));
+        yv[q] = *reinterpret_cast<const Vec*>(ybar + (ok ? bout + oo[q] : 0));
+        if (!ok) {
+          xv[q] = Vec(0);
+          yv[q] = Vec(0);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < VN; ++e)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+            dacc[rb][cb] = Mfma<T>::op(yv[rb][e], xv[cb][e], dacc[rb][cb]);
+    }
+  };
+  for (uint32_t t = wid; t < full; t += nw) tile(t, false);
+  if (full * COLS < n_orbits && wid == full % nw) tile(full, true);
+
+  __syncthreads();  // every wave is done with the PTM: its LDS now sums dP
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb) {
+#pragma unroll
+    for (int ww = 0; ww < kBlock / 64; ++ww) {
+      if (w == ww) {
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = sizeof(T) == 8 ? kq + 4 * r : 4 * kq + r;
+            const int idx = (16 * rb + row) * 64 + 16 * cb + c;
+            smem[idx] = (ww == 0 ? 0.0 : smem[idx]) + double(dacc[rb][cb][r]);
+          }
+      }
+      __syncthreads();
+    }
+  }
+  for (int jj = tid; jj < 64 * 64; jj += kBlock)
+    partial[(int64_t(b) * gridDim.x + blockIdx.x) * 64 * 64 + jj] = smem[jj];
+}
+
+// Offsets of the transposed apply (input: ybar, laid out like the forward's
+// output; output: xbar, like the forward's input), vector widths 1 along
+// the PTM index.
+static FusedOffs bwd64_offsets(const GradPlan& p) {
+  FusedOffs fo;
+  memset(&fo, 0, sizeof(fo));
+  for (int u = 0; u < 16; ++u) fo.in_u[u] = p.offs.off_out[fused_k(1, u, 0)];
+  for (int v = 0; v < 16; ++v) fo.out_u[v] = p.offs.off_in[fused_o(1, v, 0)];
+  for (int j = 0; j < 4; ++j) {
+    fo.in_l[j] = p.offs.off_out[fused_k(1, 0, j)];
+    fo.out_l[j] = p.offs.off_in[fused_o(1, 0, j)];
+  }
+  return fo;
+}
+
+// The fused backward applies: square 64 x 64 PTM, offsets additive over the
+// index bits (as the MFMA maps need), orbit vectors (p.vn > 1).
+static bool bwd64_applies(const GradPlan& p) {
+  if (p.din != 64 || p.dout != 64 || p.vn <= 1) return false;
+  for (int k = 0; k < 64; ++k) {
+    int64_t si = 0, so = 0;
+    for (int bit = 1; bit < 64; bit <<= 1)
+      if (k & bit) {
+        si += p.offs.off_in[bit];
+        so += p.offs.off_out[bit];
+      }
+    if (si != p.offs.off_in[k] || so != p.offs.off_out[k]) return false;
+  }
+  return true;
+}
+
 }  // namespace qsv
 
 using namespace qsv;
@@ -490,8 +678,8 @@ extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, in
 // Host replay of k_ptm_grad64 + k_ptm_grad_final for one state (same checks
 // as the replay of the 16 x 16 kernel below).
 static int emulate_grad64(const GradPlan& p, int es, const double* x, const double* ybar, int nb,
-                          double* dptm) {
-  const int VN = p.vn, waves = kBlock / 64, U = kGrad64Unroll;
+                          double* dptm, int U = kGrad64Unroll) {
+  const int VN = p.vn, waves = kBlock / 64;
   const uint32_t STEP = 4 * VN, nsteps = (p.n_orbits + STEP - 1) / STEP;
   const uint32_t nchunks = (nsteps + U - 1) / U;
   const int dd = p.din * p.dout;
@@ -703,4 +891,195 @@ extern "C" int qsv_emulate_ptm_grad(int dtype, const double* x, const double* yb
     dptm[j] = s;
   }
   return QSV_OK;
+}
+
+// ------------------------------------------------------ fused backward
+static int bwd64_plan(int dtype, int rank, const int64_t* dims, int k, const int32_t* targets,
+                      GradPlan& p) {
+  QSV_CHECK_ARG(dtype == QSV_F32 || dtype == QSV_F64, "unknown dtype %d", dtype);
+  QSV_CHECK_ARG(dims && targets && k >= 1 && k <= QSV_MAX_TARGETS && rank >= 1 &&
+                    rank <= QSV_MAX_RANK,
+                "bad shape arguments");
+  int32_t dout[QSV_MAX_TARGETS];
+  for (int j = 0; j < k; ++j) {
+    QSV_CHECK_ARG(targets[j] >= 0 && targets[j] < rank, "bad target %d", targets[j]);
+    dout[j] = int32_t(dims[targets[j]]);
+  }
+  const int rc = make_grad_plan(dtype_size(dtype), rank, dims, k, targets, dout, p);
+  if (rc) return rc;
+  if (!bwd64_applies(p)) {
+    set_error("fused backward needs a square 64 x 64 PTM and a unit-stride bystander "
+              "innermost axis");
+    return QSV_ENOTSUP;
+  }
+  return QSV_OK;
+}
+
+static int bwd64_blocks(const GradPlan& p, int batch, uint32_t cap) {
+  const uint32_t tiles = (p.n_orbits + 16 * p.vn - 1) / (16 * p.vn);
+  uint32_t nb = (tiles + kBlock / 64 - 1) / (kBlock / 64);
+  nb = std::min<uint32_t>(nb, std::max<uint32_t>(1, cap / std::max(batch, 1)));
+  nb = std::min<uint32_t>(nb, kGrad64Blocks);
+  return int(std::max<uint32_t>(nb, 1));
+}
+
+template <typename T, int VN>
+static uint32_t bwd64_cap() {
+  static const uint32_t cap = [] {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(k_bwd64<T, VN>), kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+      per_cu = 1;
+    (void)hipGetLastError();
+    return uint32_t(per_cu) * uint32_t(cus);
+  }();
+  return cap;
+}
+
+extern "C" int qsv_apply_ptm_batched_backward(int dtype, const void* x, const void* ybar,
+                                              void* xbar, int batch, int rank,
+                                              const int64_t* dims, int k,
+                                              const int32_t* targets, const void* ptm_dev,
+                                              int64_t ptm_batch_stride, void* dptm,
+                                              int accumulate, void* workspace,
+                                              int64_t workspace_bytes, void* stream) {
+  QSV_CHECK_ARG(batch >= 0 && batch <= 65535, "batch %d outside [0, 65535]", batch);
+  QSV_CHECK_ARG(ptm_batch_stride >= 0, "negative PTM batch stride");
+  GradPlan p;
+  const int rc = bwd64_plan(dtype, rank, dims, k, targets, p);
+  if (rc) return rc;
+  if (batch == 0) return QSV_OK;
+  QSV_CHECK_ARG(x && ybar && xbar && ptm_dev && dptm, "null pointer argument");
+  QSV_CHECK_ARG(xbar != ybar && xbar != x, "xbar must not alias x or ybar");
+  const int es = dtype_size(dtype);
+  auto al = [](const void* q) { return reinterpret_cast<uintptr_t>(q) % 16 == 0; };
+  if (!al(x) || !al(ybar) || !al(xbar) || (p.n_in * es) % 16) {
+    set_error("fused backward needs 16-byte aligned states");
+    return QSV_ENOTSUP;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const FusedOffs fo = bwd64_offsets(p);
+  const uint32_t cap = dtype == QSV_F64 ? bwd64_cap<double, 2>() : bwd64_cap<float, 4>();
+  const int nb = bwd64_blocks(p, batch, cap);
+  const int64_t need = int64_t(batch) * nb * 64 * 64 * 8;
+  QSV_CHECK_ARG(workspace && workspace_bytes >= need, "fused backward needs %lld workspace bytes",
+                (long long)need);
+  double* partial = static_cast<double*>(workspace);
+  const dim3 grid(nb, batch);
+  if (dtype == QSV_F64)
+    hipLaunchKernelGGL((k_bwd64<double, 2>), grid, dim3(kBlock), 0, s,
+                       static_cast<const double*>(x), static_cast<const double*>(ybar),
+                       static_cast<double*>(xbar), p.g, p.n_orbits,
+                       static_cast<const double*>(ptm_dev), ptm_batch_stride, p.offs, fo, partial);
+  else
+    hipLaunchKernelGGL((k_bwd64<float, 4>), grid, dim3(kBlock), 0, s,
+                       static_cast<const float*>(x), static_cast<const float*>(ybar),
+                       static_cast<float*>(xbar), p.g, p.n_orbits,
+                       static_cast<const float*>(ptm_dev), ptm_batch_stride, p.offs, fo, partial);
+  QSV_CHECK_HIP(hipGetLastError());
+  const dim3 g2((64 * 64 + kBlock - 1) / kBlock, batch);
+  const double acc_flag = accumulate ? 1.0 : 0.0;
+  if (dtype == QSV_F64)
+    hipLaunchKernelGGL(k_ptm_grad_final<double>, g2, dim3(kBlock), 0, s, partial, nb, 64 * 64,
+                       static_cast<double*>(dptm), acc_flag);
+  else
+    hipLaunchKernelGGL(k_ptm_grad_final<float>, g2, dim3(kBlock), 0, s, partial, nb, 64 * 64,
+                       static_cast<float*>(dptm), acc_flag);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
+// Host replay of k_bwd64 + k_ptm_grad_final for one state.  Part A: every
+// lane's loads and stores (bounds, vector alignment, each xbar entry written
+// once) and the MFMA contraction in hardware index order with the
+// transposed PTM's fragment table; part B: the wide-gradient replay with
+// the kernel's four k-steps per tile.
+extern "C" int qsv_emulate_apply_ptm_batched_backward(int dtype, const double* x,
+                                                      const double* ybar, double* xbar,
+                                                      int rank, const int64_t* dims, int k,
+                                                      const int32_t* targets,
+                                                      const double* ptm, double* dptm,
+                                                      int32_t grad_blocks) {
+  GradPlan p;
+  const int rc = bwd64_plan(dtype, rank, dims, k, targets, p);
+  if (rc) return rc;
+  QSV_CHECK_ARG(x && ybar && xbar && ptm && dptm, "null pointer argument");
+  QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24), "emulation is for small vectors");
+  QSV_CHECK_ARG(grad_blocks >= 0 && grad_blocks <= kGrad64Blocks, "grad_blocks outside [0, %d]",
+                kGrad64Blocks);
+  const int es = dtype_size(dtype), VN = p.vn, MB = 4, KS = 16;
+  const int nb = grad_blocks > 0 ? grad_blocks : bwd64_blocks(p, 1, 256);
+  const FusedOffs fo = bwd64_offsets(p);
+  // fragment table of P^T, as the kernel gathers it
+  std::vector<double> As(KS * MB * 64);
+  for (int i = 0; i < KS * MB * 64; ++i) {
+    int o, kk;
+    fused_frag_coords(64, es, 1, 1, i, o, kk);
+    QSV_CHECK_ARG(o >= 0 && o < 64 && kk >= 0 && kk < 64, "bwd64: fragment map out of range");
+    As[i] = ptm[int64_t(kk) * 64 + o];
+  }
+  const uint32_t COLS = 16 * VN, ntiles = (p.n_orbits + COLS - 1) / COLS;
+  std::vector<int> written(p.n_in, 0);
+  std::vector<double> bv(size_t(64) * VN * KS), acc(size_t(64) * MB * VN * 4);
+  for (uint32_t t = 0; t < ntiles; ++t) {
+    for (int lane = 0; lane < 64; ++lane) {
+      const int n = lane & 15, j = lane >> 4;
+      const uint32_t m0 = t * COLS + VN * n;
+      const uint32_t m = m0 < p.n_orbits ? m0 : p.n_orbits - VN;
+      int64_t bin, bout;
+      orbit_base(p.g, m, bin, bout);
+      for (int e = 1; e < VN; ++e) {
+        int64_t bi2, bo2;
+        orbit_base(p.g, m + e, bi2, bo2);
+        QSV_CHECK_ARG(bi2 == bin + e && bo2 == bout + e, "bwd64: orbits of a vector not adjacent");
+      }
+      for (int s = 0; s < KS; ++s) {
+        const int64_t a = bout + fo.in_l[j] + fo.in_u[s];
+        QSV_CHECK_ARG(a % VN == 0 && a >= 0 && a + VN <= p.n_out, "bwd64: ybar load %lld",
+                      (long long)a);
+        for (int e = 0; e < VN; ++e) bv[(size_t(lane) * VN + e) * KS + s] = ybar[a + e];
+      }
+    }
+    // acc[lane][mb][nb][r] = sum_s sum_kk A[row][kk] B[kk][col]
+    std::fill(acc.begin(), acc.end(), 0.0);
+    for (int s = 0; s < KS; ++s)
+      for (int mb = 0; mb < MB; ++mb)
+        for (int e = 0; e < VN; ++e)
+          for (int lane = 0; lane < 64; ++lane)
+            for (int r = 0; r < 4; ++r) {
+              const int jj = lane >> 4, col = lane & 15;
+              const int row = es == 8 ? jj + 4 * r : 4 * jj + r;
+              double d = 0.0;
+              for (int kk = 0; kk < 4; ++kk)
+                d += As[(s * MB + mb) * 64 + row + 16 * kk] *
+                     bv[(size_t(col + 16 * kk) * VN + e) * KS + s];
+              acc[((size_t(lane) * MB + mb) * VN + e) * 4 + r] += d;
+            }
+    for (int lane = 0; lane < 64; ++lane) {
+      const int n = lane & 15, j = lane >> 4;
+      const uint32_t m0 = t * COLS + VN * n;
+      if (m0 >= p.n_orbits) continue;
+      int64_t bin, bout;
+      orbit_base(p.g, m0, bin, bout);
+      for (int q = 0; q < 4 * MB; ++q) {
+        const int64_t a = bin + fo.out_l[j] + fo.out_u[q];
+        QSV_CHECK_ARG(a % VN == 0 && a >= 0 && a + VN <= p.n_in, "bwd64: xbar store %lld",
+                      (long long)a);
+        for (int e = 0; e < VN; ++e) {
+          xbar[a + e] = acc[((size_t(lane) * MB + (q >> 2)) * VN + e) * 4 + (q & 3)];
+          ++written[a + e];
+        }
+      }
+    }
+  }
+  for (int64_t a = 0; a < p.n_in; ++a)
+    QSV_CHECK_ARG(written[a] == 1, "bwd64: xbar entry %lld written %d times", (long long)a,
+                  written[a]);
+  // part B: per tile four k-steps of 4 VN orbits, chunks strided like the tiles
+  return emulate_grad64(p, es, x, ybar, nb, dptm, 4);
 }

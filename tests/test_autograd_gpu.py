@@ -333,6 +333,9 @@ def test_fused_batch_equals_unfused(dtype):
     for a, b in zip(ge, gf):
         scale = b.abs().max().item() + 1e-30
         assert (a - b).abs().max().item() < (1e-12 if dtype == torch.float64 else 1e-5) * scale
+    # the graph's backward of the 3-qubit blocks runs the one-pass kernel,
+    # the eager step the two separate kernels: float32 dL/dP sums partition
+    # differently
     for rep in range(3):
         torch.randn(1000, device=DEV).sum().item()
         _, pg, gg = fused.replay()
@@ -340,7 +343,9 @@ def test_fused_batch_equals_unfused(dtype):
         for a, b in zip(pg, pe):
             assert torch.allclose(a, b, rtol=1e-12, atol=1e-14), rep
         for a, b in zip(gg, ge):
-            assert torch.allclose(a, b, rtol=1e-6, atol=1e-9), rep
+            scale = b.abs().max().item() + 1e-30
+            assert (a - b).abs().max().item() < (1e-11 if dtype == torch.float64
+                                                 else 1e-5) * scale, rep
 
 
 def test_fused_batch_matches_oracle():
@@ -373,3 +378,37 @@ def test_trainable_circuit_fused_gradient():
     assert abs(out[0][0] - out[1][0]) < 1e-12
     for a, b in zip(out[0][1], out[1][1]):
         assert torch.allclose(a, b, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12), (torch.float32, 1e-5)])
+@pytest.mark.parametrize('n,targets', [(6, (1, 2, 3)), (6, (0, 1, 2)), (6, (3, 0, 2)),
+                                       (10, (4, 5, 6)), (10, (7, 8, 9)), (5, (2, 3, 4))])
+def test_fused_backward_matches_separate_kernels(n, targets, dtype, tol):
+    """One-pass backward of dense 64 x 64 blocks (qsv_apply_ptm_batched_backward)
+    vs the transposed apply (bitwise: the same MFMA maps) and the PTM
+    gradient kernel; per-state and shared PTMs; geometries where it does not
+    apply (innermost axis a target) fall back to the separate kernels."""
+    from quantumsim_amd.autograd import _backward_raw, _apply_raw, _ptm_grad_raw
+    rng = np.random.default_rng(n + sum(targets))
+    B = 3
+    shape = (4,) * n
+    print('fused backward', n, targets, dtype, flush=True)
+    x = torch.tensor(rng.standard_normal((B, 4 ** n)), dtype=dtype, device=DEV)
+    gy = torch.tensor(rng.standard_normal((B, 4 ** n)), dtype=dtype, device=DEV)
+    for P in (torch.tensor(rng.standard_normal((B, 64, 64)) / 8, dtype=dtype, device=DEV),
+              torch.tensor(rng.standard_normal((1, 64, 64)) / 8, dtype=dtype, device=DEV)):
+        gx, gp = _backward_raw(x, gy, P, shape, targets, True, fused=True)
+        gx2 = _apply_raw(gy, P.transpose(1, 2), shape, targets, [4, 4, 4])
+        gp2 = _ptm_grad_raw(x, gy, shape, targets, [4, 4, 4])
+        assert torch.equal(gx, gx2)
+        err = ((gp.double() - gp2.double()).abs().max() / gp2.double().abs().max()).item()
+        assert err < tol, err
+        xn = x.double().cpu().numpy().reshape((B,) + shape)
+        yn = gy.double().cpu().numpy().reshape((B,) + shape)
+        lx = [n + 1] + list(range(n))
+        ly = [n + 1] + list(range(n))
+        for j, t in enumerate(targets):
+            ly[1 + t] = n + 2 + j
+        ref = np.einsum(yn, ly, xn, lx, [n + 1] + [n + 2 + j for j in range(3)] + list(targets))
+        err = np.max(np.abs(gp.double().cpu().numpy() - ref.reshape(B, 64, 64))) / np.max(np.abs(ref))
+        assert err < tol, err

@@ -330,3 +330,54 @@ def test_grad64_load_modes():
         assert emulate_grad(dtype, x, x, (2, 3, 4), [4, 4, 4])[1] == 1 | 256
         assert emulate_grad(dtype, x, x, (4, 2, 3), [4, 4, 4])[1] == 1
         assert emulate_grad(dtype, x, x, (0, 1, 4), [4, 4, 4])[1] == 1 | 256
+
+
+def emulate_backward(dtype, x, ybar, ptm, targets, grad_blocks=0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    ybar = np.ascontiguousarray(ybar, dtype=np.float64)
+    P = np.ascontiguousarray(ptm, dtype=np.float64).reshape(64, 64)
+    xbar = np.zeros_like(x)
+    dptm = np.zeros(64 * 64)
+    rc = nat.lib().qsv_emulate_apply_ptm_batched_backward(
+        dtype, x.ctypes.data_as(nat._c_f64p), ybar.ctypes.data_as(nat._c_f64p),
+        xbar.ctypes.data_as(nat._c_f64p), x.ndim, nat.i64(x.shape), len(targets),
+        nat.i32(targets), P.ctypes.data_as(nat._c_f64p), dptm.ctypes.data_as(nat._c_f64p),
+        grad_blocks)
+    if rc == -3:
+        return None
+    nat.check(rc, 'emulate_backward')
+    return xbar, dptm.reshape(64, 64)
+
+
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets', [([4] * 5, (1, 2, 3)), ([4] * 6, (0, 1, 2)),
+                                          ([4] * 5, (3, 0, 2)), ([4] * 4, (0, 1, 2)),
+                                          ([4] * 6, (4, 1, 2))])
+def test_emulated_fused_backward_matches_oracle(dims, targets, dtype, rng):
+    """The fused backward of a dense 64 x 64 block (xbar = P^T ybar and dP
+    in one pass), replayed: both parts against numpy."""
+    n = len(dims)
+    x = rng.standard_normal(dims)
+    ybar = rng.standard_normal(dims)
+    P = rng.standard_normal((4,) * 6)
+    ref_xbar = oracle_apply(ybar, np.ascontiguousarray(
+        P.reshape(64, 64).T).reshape((4,) * 6), targets)
+    xl, yl = list(range(n)), list(range(n))
+    for j, t in enumerate(targets):
+        yl[t] = n + j
+    ref_dp = np.einsum(ybar, yl, x, xl, [n + j for j in range(3)] + list(targets)).reshape(64, 64)
+    for blocks in (0, 1, 3):
+        r = emulate_backward(dtype, x, ybar, P, targets, blocks)
+        assert r is not None
+        xbar, dp = r
+        assert np.allclose(xbar, ref_xbar, rtol=1e-12, atol=1e-12), (dims, targets)
+        assert np.allclose(dp, ref_dp, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
+
+
+def test_fused_backward_not_applicable():
+    """Innermost axis a target (or a non-64 PTM): QSV_ENOTSUP, callers fall
+    back to the separate kernels."""
+    x = np.zeros([4] * 5)
+    P = np.zeros((4,) * 6)
+    assert emulate_backward(nat.QSV_F32, x, x, P, (2, 3, 4)) is None
+    assert emulate_backward(nat.QSV_F64, x, x, P, (0, 1, 4)) is None
