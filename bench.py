@@ -131,7 +131,11 @@ def bench_batched(args):
     t_setup = time.perf_counter() - t0
     stream = cb.stream
     reps = 3
-    sec = timed(lambda: cb.graph.replay(), reps, stream)
+    # CUDAGraph.replay() launches on the current stream: make it the batch's
+    # stream, the one the events are recorded on (otherwise the events time
+    # only the host-side launch of the replays)
+    with torch.cuda.stream(stream):
+        sec = timed(lambda: cb.graph.replay(), reps, stream)
     log('batched: %d circuits fwd+adjoint %.3f s per step (setup+capture %.1f s)'
         % (len(circuits), sec, t_setup))
     # plain baseline: each circuit alone, eager launches, same kernels

@@ -88,23 +88,18 @@ def _ptm_grad_raw(x, ybar, shape, targets, dims_out):
     return out
 
 
-def _backward_raw(x, gy, ptm, shape, targets, need_ptm_grad, fused=None):
+def _backward_raw(x, gy, ptm, shape, targets, need_ptm_grad, fused=False):
     """(dL/dx, dL/dP or None) of y = P x on the states x [B, N] for the
-    incoming gy [B, N].
+    incoming gy [B, N]: the transposed apply and the PTM gradient kernel.
 
-    Square dense 64 x 64 PTMs recorded into a HIP graph use the one-pass
-    backward (qsv_apply_ptm_batched_backward: two graph nodes per gate
-    instead of four); everywhere else the transposed apply and the PTM
-    gradient kernel.  Measured on MI355X: a captured graph's replay time
-    follows its node count once nodes are short (two extra tiny nodes per
-    backward gate add 0.10 s to the 512-circuit step), and the one-pass
-    backward makes that step 1.5x faster; launched eagerly, each of the two
-    separate kernels runs at higher occupancy and they are faster together
-    (3.7 vs 5.0 ms for 64 twelve-qubit states).  ``fused`` overrides the
-    choice (True: one pass where the kernel applies; False: never)."""
+    ``fused=True`` uses the one-pass backward of square dense 64 x 64 PTMs
+    instead (qsv_apply_ptm_batched_backward, where it applies).  It reads
+    one stream less but runs at two waves per SIMD; measured on MI355X it is
+    slower than the two separate kernels (5.0 vs 3.7 ms for 64
+    twelve-qubit states; 0.44 vs 0.41 s of GPU time for the 512-circuit
+    step), so it is not the default.
+    """
     d = [shape[q] for q in targets]
-    if fused is None:
-        fused = torch.cuda.is_current_stream_capturing()
     if fused and need_ptm_grad and ptm.shape[-1] == 64 and ptm.shape[-2] == 64:
         B = x.shape[0]
         L = nat.lib()
