@@ -13,7 +13,8 @@ are streamed at HBM bandwidth anyway) are grouped by qubit count; inside a
 group the states form one [B, 4^n] tensor ordered by decreasing depth, so
 the circuits still running at any layer are a prefix of the batch.  Layers
 are taken two at a time and their gates composed (torch, differentiable)
-into dense blocks of up to three qubits, each one qsv_apply_ptm_batched
+into n // 2 dense blocks of up to three qubits (workloads.brickwork_window_plan),
+each one qsv_apply_ptm_batched
 launch over that prefix (64 x 64 PTMs on the MFMA kernel): about half the
 passes over the states of one launch per gate.  A circuit that ends inside
 a two-layer window gets identity PTMs for its missing layer; circuits that
@@ -36,7 +37,8 @@ from .autograd import Gate, run_circuit, diagonal
 from .autotune import tuning
 from .bases import general
 from .fusion import fuse_batched
-from .workloads import rotation_ptms, damping_ptms, fused_layer_gates, layer_targets
+from .workloads import (rotation_ptms, damping_ptms, fused_layer_gates, layer_targets,
+                        brickwork_window_plan)
 
 __all__ = ['CircuitBatch', 'plain_forward_backward']
 
@@ -227,7 +229,10 @@ class CircuitBatch:
                     ptm = torch.where(keep, ptm, eye)
                 gates.append(Gate(ptm, gt.targets))
         out = []
-        for m, q in fuse_batched(gates, [4] * n, self.fuse_qudits):
+        # the right-to-left brickwork tiling (n // 2 blocks of up to three
+        # qubits) when blocks of three are allowed, else the greedy pass
+        plan = brickwork_window_plan(n) if self.fuse_qudits == 3 else None
+        for m, q in fuse_batched(gates, [4] * n, self.fuse_qudits, plan=plan):
             D = m.shape[-1]
             out.append((m.reshape(B, nw, D, D).transpose(0, 1).to(self.dtype).contiguous(), q))
         return out

@@ -182,12 +182,15 @@ def embed_ptm_batched(ptm, qudits, block, dims):
     return out.reshape(B, D, D)
 
 
-def fuse_batched(gates, dims, max_qudits=3):
+def fuse_batched(gates, dims, max_qudits=3, plan=None):
     """Fuse batched gates (objects with ``.ptm`` [B or 1, D, D] torch
     tensors and ``.targets``) into dense blocks of up to ``max_qudits``
-    qudits with torch ops, so autograd reaches the source PTMs.  Returns
-    [(ptm [B, Db, Db], block qudits), ...] in application order."""
-    plan = fuse_plan([g.targets for g in gates], dims, max_qudits)
+    qudits with torch ops, so autograd reaches the source PTMs.  ``plan``:
+    [(block qudits, source indices in order), ...] to use instead of
+    ``fuse_plan``'s.  Returns [(ptm [B, Db, Db], block qudits), ...] in
+    application order."""
+    if plan is None:
+        plan = fuse_plan([g.targets for g in gates], dims, max_qudits)
     out = []
     for block, src in plan:
         mat = None

@@ -150,6 +150,38 @@ def layer_targets(n, layer):
     return [tuple(pq) for pq in pairs] + [(q,) for q in range(n) if q not in paired]
 
 
+def brickwork_window_plan(n):
+    """Dense blocks of two consecutive brickwork layers (the gates of
+    layer_targets(n, 0) followed by those of layer_targets(n, 1), indexed in
+    that order): [(block qubits, source gate indices in application order),
+    ...] in application order.
+
+    Block k covers qubits 2k, 2k+1, 2k+2 and holds the first layer's bond
+    (2k, 2k+1) and the second layer's bond (2k+1, 2k+2) (plus the single-qubit
+    gates at the ends); the blocks run from the highest k down, so the
+    second-layer bond (2k+1, 2k+2) comes after the first-layer bond
+    (2k+2, 2k+3) of block k+1 that shares its qubit.  ceil((n-1)/2) blocks,
+    one fewer than the greedy fusion pass for odd n."""
+    ta, tb = layer_targets(n, 0), layer_targets(n, 1)
+    ia = {t: i for i, t in enumerate(ta)}
+    ib = {t: len(ta) + i for i, t in enumerate(tb)}
+    blocks = []
+    for k in reversed(range(max(1, n // 2))):
+        qs = [q for q in (2 * k, 2 * k + 1, 2 * k + 2) if q < n]
+        src = []
+        for t in ((2 * k, 2 * k + 1), (2 * k + 2,)):
+            if t in ia:
+                src.append(ia[t])
+        for t in ((2 * k + 1, 2 * k + 2),):
+            if t in ib:
+                src.append(ib[t])
+        for q in qs:
+            if (q,) in ib and (q == 0 or q == n - 1):
+                src.append(ib[(q,)])
+        blocks.append((tuple(qs), src))
+    return blocks
+
+
 def fused_layer_gates(n, layer, rot, damp):
     """Gates of one layer for a batch: rot, damp [B, n, 4, 4] PTMs."""
     from .autograd import Gate, cz_ptm, kron_ptm

@@ -147,3 +147,27 @@ def test_fuse_batched_gradient_reaches_sources():
         m, _ = fuse_batched([G(a_, (0, 1)), G(b_, (1, 2))], [4, 4, 4], 3)[0]
         return (m ** 2).sum()
     assert torch.autograd.gradcheck(f, (a, b), eps=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize('n', list(range(2, 13)))
+def test_brickwork_window_plan(n):
+    """The right-to-left tiling of two brickwork layers: every gate once,
+    blocks of at most 3 qubits, n // 2 blocks, and the blocks applied in
+    order equal the gates applied in order (oracle)."""
+    from quantumsim_amd.workloads import brickwork_window_plan, layer_targets
+    rng = np.random.default_rng(n)
+    tg = layer_targets(n, 0) + layer_targets(n, 1)
+    plan = brickwork_window_plan(n)
+    assert len(plan) == max(1, n // 2)
+    assert sorted(i for _, s in plan for i in s) == list(range(len(tg)))
+    gates = [(rng.standard_normal([4] * (2 * len(t))) / 4 ** len(t), t) for t in tg]
+    blocks = []
+    for qs, src in plan:
+        assert len(qs) <= 3 and all(set(tg[i]) <= set(qs) for i in src)
+        m = np.eye(4 ** len(qs))
+        for i in src:
+            m = embed_ptm(gates[i][0], gates[i][1], qs, [4] * n) @ m
+        blocks.append((m.reshape([4] * (2 * len(qs))), qs))
+    x0 = rng.standard_normal([4] * n)
+    bl = [ob.general(2)] * n
+    np.testing.assert_allclose(_apply_all(bl, blocks, x0), _apply_all(bl, gates, x0), atol=1e-12)
