@@ -13,12 +13,13 @@ are streamed at HBM bandwidth anyway) are grouped by qubit count; inside a
 group the states form one [B, 4^n] tensor ordered by decreasing depth, so
 the circuits still running at any layer are a prefix of the batch.  Layers
 are taken two at a time and their gates composed (torch, differentiable)
-into n // 2 dense blocks of up to three qubits (workloads.brickwork_window_plan),
-each one qsv_apply_ptm_batched
-launch over that prefix (64 x 64 PTMs on the MFMA kernel): about half the
-passes over the states of one launch per gate.  A circuit that ends inside
-a two-layer window gets identity PTMs for its missing layer; circuits that
-have finished are retired by narrowing the prefix, without copies.  Forward, the float64 population readout and the adjoint
+into n // 2 dense blocks of up to three qubits
+(workloads.brickwork_window_plan), each one qsv_apply_ptm_batched launch
+over that prefix (64 x 64 PTMs on the MFMA kernel): about half the passes
+over the states of one launch per gate.  A circuit that ends inside a
+two-layer window gets identity PTMs for its missing layer; circuits that
+have finished are retired by narrowing the prefix, without copies.
+Forward, the float64 population readout and the adjoint
 (checkpointed recompute + qsv_ptm_grad) are all device work with no host
 synchronisation, so the whole step can be captured once in a HIP graph and
 replayed (``CircuitBatch.capture``).
