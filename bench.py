@@ -124,11 +124,15 @@ def bench_batched(args):
     tuner = Autotuner(path=None)                 # tuned in this run, not read from disk
     set_tuner(tuner)
     t0 = time.perf_counter()
-    cb = CircuitBatch(circuits, dtype=torch.float32, checkpoint_every=16,
-                      fuse_qudits=args.fuse_qudits)
+    torch.cuda.reset_peak_memory_stats()
+    free0 = torch.cuda.mem_get_info()[0]
+    # checkpoint interval chosen from the memory budget (every state kept
+    # when the states fit 60 % of the free device memory)
+    cb = CircuitBatch(circuits, dtype=torch.float32, fuse_qudits=args.fuse_qudits)
     w = cb.random_weights(7)
     cb.capture(w, autotune=True)
     t_setup = time.perf_counter() - t0
+    peak_gb = torch.cuda.max_memory_allocated() / 1e9
     stream = cb.stream
     reps = 3
     # CUDAGraph.replay() launches on the current stream: make it the batch's
@@ -184,7 +188,9 @@ def bench_batched(args):
            'plain_circuits_per_s_by_fuse': {str(k): round(len(circuits) / v, 2)
                                             for k, v in plain_by_fuse.items()},
            'dtype': 'f32 state, f64 readout',
-           'checkpoint_every': 16, 'setup_s': round(t_setup, 1),
+           'checkpoint_every': cb.k, 'kept_state_gb': round(cb.kept_bytes(cb.k) / 1e9, 1),
+           'peak_alloc_gb': round(peak_gb, 1), 'free_gb_at_start': round(free0 / 1e9, 1),
+           'setup_s': round(t_setup, 1),
            'max_prob_err_vs_f64': float('%.3e' % prob_err), 'fuse_qudits': args.fuse_qudits,
            'autotune': tuner.summary()}
     log('batched: %s' % res)
@@ -239,6 +245,7 @@ def bench_differentiable(args):
     from quantumsim_amd.workloads import TrainableCircuit
     tc = TrainableCircuit(n=12, layers=40)
     sets = args.param_sets
+    free0 = torch.cuda.mem_get_info()[0]
     ang, t1, t2 = tc.initial_params(sets, seed=11, device='cuda', dtype=torch.float64)
     with torch.no_grad():
         target = tc.populations(ang + 0.1, t1 * 1.05, t2 * 0.97, torch.float32)
@@ -260,7 +267,7 @@ def bench_differentiable(args):
     res = {'qubits': 12, 'layers': 40, 'params_per_set': tc.n_params, 'param_sets': sets,
            'fwd_adjoint_s': round(sec, 4), 'param_sets_per_s': round(sets / sec, 2),
            'gates': n_gates, 'applications': n_apps, 'fuse_qudits': args.fuse_qudits,
-           'checkpoint_every': 8}
+           'checkpoint_every': 8, 'free_gb_at_start': round(free0 / 1e9, 1)}
     log('differentiable: %s' % res)
     return res
 

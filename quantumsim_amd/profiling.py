@@ -77,8 +77,7 @@ def main(argv=None):
     if a.what == 'batched':
         from quantumsim_amd.batch import CircuitBatch
         from quantumsim_amd.workloads import random_circuit_batch
-        cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32,
-                          checkpoint_every=16)
+        cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32)
         w = cb.random_weights(7)
         cb.capture(w, autotune=True)       # tuning + warm-up + capture
         torch.cuda.synchronize()

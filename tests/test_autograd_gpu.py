@@ -411,3 +411,29 @@ def test_fused_backward_matches_separate_kernels(n, targets, dtype, tol):
         ref = np.einsum(yn, ly, xn, lx, [n + 1] + [n + 2 + j for j in range(3)] + list(targets))
         err = np.max(np.abs(gp.double().cpu().numpy() - ref.reshape(B, 64, 64))) / np.max(np.abs(ref))
         assert err < tol, err
+
+
+def test_circuit_batch_checkpoint_interval_from_budget():
+    """The interval comes from the memory budget (every state kept when it
+    fits, the longest interval when nothing does), and it changes only the
+    memory: populations bitwise, gradients to 1e-12, graph replays too."""
+    circuits = random_circuit_batch(8, seed=21, n_range=(7, 9), layer_range=(3, 11))
+    keep_all = CircuitBatch(circuits, dtype=torch.float64, checkpoint_budget=1e15)
+    recompute = CircuitBatch(circuits, dtype=torch.float64, checkpoint_budget=0)
+    assert keep_all.k == 1 and recompute.k == 16
+    assert keep_all.kept_bytes(1) > recompute.kept_bytes(16)
+    w = keep_all.random_weights(3)
+    _, p1, g1 = keep_all.step(w)
+    _, p2, g2 = recompute.step(w)
+    for a, b in zip(p1, p2):
+        assert torch.equal(a, b)
+    for a, b in zip(g1, g2):
+        assert torch.allclose(a, b, rtol=1e-12, atol=1e-14)
+    keep_all.capture(w, autotune=False)
+    for _ in range(2):
+        _, pr, gr = keep_all.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(pr, p1):
+            assert torch.equal(a, b)
+        for a, b in zip(gr, g1):
+            assert torch.allclose(a, b, rtol=1e-12, atol=1e-14)
