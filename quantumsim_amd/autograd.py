@@ -30,7 +30,7 @@ from . import _native as nat
 from .autotune import get_tuner
 from .bases import general
 
-__all__ = ['apply', 'run_circuit', 'diagonal', 'Gate', 'qubit_basis_tensor', 'kraus_ptm',
+__all__ = ['apply', 'run_circuit', 'run_schedule', 'diagonal', 'Gate', 'qubit_basis_tensor', 'kraus_ptm',
            'rx_ptm', 'rz_ptm', 'damping_ptm', 'cz_ptm', 'kron_ptm', 'chain_ptm']
 
 
@@ -51,12 +51,17 @@ class _Workspace:
         return t
 
 
-def _apply_raw(x, ptm, shape, targets, dims_out):
-    """y[b] = ptm[b or 0] applied to x[b] on `targets` (no autograd)."""
+def _apply_raw(x, ptm, shape, targets, dims_out, out=None):
+    """y[b] = ptm[b or 0] applied to x[b] on `targets` (no autograd); into
+    ``out`` (contiguous, x's shape and type) when given."""
     B = x.shape[0]
     n_out = math.prod(shape[q] if q not in targets else dims_out[targets.index(q)]
                       for q in range(len(shape)))
-    y = torch.empty((B, n_out), dtype=x.dtype, device=x.device)
+    if out is None:
+        y = torch.empty((B, n_out), dtype=x.dtype, device=x.device)
+    else:
+        assert out.shape == (B, n_out) and out.dtype == x.dtype and out.is_contiguous()
+        y = out
     ptm = ptm.to(dtype=x.dtype).contiguous()
     pstride = 0 if ptm.shape[0] == 1 else ptm.shape[1] * ptm.shape[2]
     tile = get_tuner().apply_tile(x.dtype, B, shape, targets, dims_out)
@@ -68,7 +73,10 @@ def _apply_raw(x, ptm, shape, targets, dims_out):
     return y
 
 
-def _ptm_grad_raw(x, ybar, shape, targets, dims_out):
+def _ptm_grad_raw(x, ybar, shape, targets, dims_out, out=None):
+    """dL/dP[b] = sum over orbits of ybar[b] x[b]^T for the states x [B, N];
+    into ``out`` (contiguous, at least B PTMs of x's type; rows past B are
+    left alone) when given."""
     B = x.shape[0]
     L = nat.lib()
     k = len(targets)
@@ -78,7 +86,11 @@ def _ptm_grad_raw(x, ybar, shape, targets, dims_out):
                                                   nat.i32(targets), nat.i32(dims_out)),
                    'ptm_grad')
     buf = _Workspace.get(x.device, ws)
-    out = torch.empty((B, dout, din), dtype=x.dtype, device=x.device)
+    if out is None:
+        out = torch.empty((B, dout, din), dtype=x.dtype, device=x.device)
+    else:
+        assert (out.shape[0] >= B and tuple(out.shape[1:]) == (dout, din)
+                and out.dtype == x.dtype and out.is_contiguous())
     tile = get_tuner().grad_tile(x.dtype, B, shape, targets, dims_out)
     rc = L.qsv_ptm_grad_tiled(nat.dtype_code(x.dtype), x.data_ptr(), ybar.data_ptr(), B,
                               len(shape), nat.i64(shape), k, nat.i32(targets), nat.i32(dims_out),
@@ -223,6 +235,108 @@ def run_circuit(x0, gates, shape, checkpoint_every=8):
     ptms = [g.ptm for g in gates]
     meta = (tuple(shape), [g.targets for g in gates], max(1, int(checkpoint_every)))
     return _Circuit.apply(x0, meta, *ptms)
+
+
+class _Schedule(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x0, meta, *steps):
+        shape, targets, segments, k = meta
+        dims = [[shape[q] for q in t] for t in targets]
+        finals = torch.empty_like(x0)
+        kept, cks = [], []
+        x = x0
+        with torch.no_grad():
+            for active, nxt, gates in segments:
+                ck = [x]
+                for j, (pos, w) in enumerate(gates):
+                    x = _apply_raw(x, steps[pos][w], shape, targets[pos], dims[pos])
+                    if (j + 1) % k == 0 and j + 1 < len(gates):
+                        ck.append(x)
+                kept.append(len(ck))
+                cks += ck
+                if nxt < active:
+                    finals[nxt:active].copy_(x[nxt:])
+                x = x[:nxt]
+        ctx.meta = meta
+        ctx.kept = kept
+        ctx.save_for_backward(*cks, *steps)
+        return finals
+
+    @staticmethod
+    def backward(ctx, gf):
+        shape, targets, segments, k = ctx.meta
+        dims = [[shape[q] for q in t] for t in targets]
+        saved = ctx.saved_tensors
+        n_ck = sum(ctx.kept)
+        cks, steps = saved[:n_ck], saved[n_ck:]
+        need = ctx.needs_input_grad[2:]
+        gf = gf.contiguous()
+        # every (position, step) PTM is used by exactly one gate: its rows
+        # of the running prefix are written in place, the rest stay zero
+        grads = [torch.zeros_like(t) if nd else None for t, nd in zip(steps, need)]
+        # transposed PTMs for the adjoint applies, one copy per position
+        steps_t = [t.transpose(-1, -2).contiguous() for t in steps]
+        bufs = [torch.empty_like(gf), torch.empty_like(gf)]
+        cur = 0
+        base = n_ck
+        for si in reversed(range(len(segments))):
+            active, nxt, gates = segments[si]
+            base -= ctx.kept[si]
+            ck = cks[base:base + ctx.kept[si]]
+            # adjoint of the segment's output: the continuing rows are in
+            # place from the later segments; the finished rows come from gf
+            if nxt < active:
+                bufs[cur][nxt:active].copy_(gf[nxt:active])
+            G = len(gates)
+            for c in reversed(range(len(ck))):
+                lo, hi = c * k, min(G, (c + 1) * k)
+                xs = [ck[c]]
+                for j in range(lo, hi - 1):
+                    pos, w = gates[j]
+                    xs.append(_apply_raw(xs[-1], steps[pos][w], shape, targets[pos], dims[pos]))
+                for j in reversed(range(lo, hi)):
+                    pos, w = gates[j]
+                    gy = bufs[cur][:active]
+                    if grads[pos] is not None:
+                        _ptm_grad_raw(xs[j - lo], gy, shape, targets[pos], dims[pos],
+                                      out=grads[pos][w])
+                    _apply_raw(gy, steps_t[pos][w], shape, targets[pos], dims[pos],
+                               out=bufs[1 - cur][:active])
+                    cur = 1 - cur
+        gx = bufs[cur].clone() if ctx.needs_input_grad[0] else None
+        return (gx, None, *grads)
+
+
+def run_schedule(x0, steps, targets, segments, shape, checkpoint_every=1):
+    """Apply a gate schedule to a batch of states whose running set shrinks:
+    circuits that share their gate structure but not their depth, sorted by
+    decreasing depth, so those still running are a prefix of the batch.
+
+    x0: states [B, N].  steps: per gate position p, the PTMs of every step
+    of that position, [n_steps_p, B, D_p, D_p] (contiguous, x0's type; step
+    w's PTM of state b is steps[p][w, b]).  targets: the qudits of each
+    position.  segments: [(active, continuing, [(p, w), ...])] in order;
+    each applies its gates to the first ``active`` states, after which the
+    states ``continuing``..``active`` - 1 are final.  Returns the final
+    states [B, N]; differentiable in x0 and every steps tensor, with a
+    checkpoint every ``checkpoint_every`` gates of a segment.
+
+    Each (p, w) must appear in at most one gate.  Against chaining
+    run_circuit calls over slices, the adjoint keeps the running prefix in
+    place (two ping-pong buffers; finished states' gradients are copied in
+    as the prefix grows) and writes dL/dP straight into one zeroed gradient
+    per position, so it makes no per-gate slice gradients, stacks or
+    concatenations.
+    """
+    seen = set()
+    for _, _, gates in segments:
+        for pw in gates:
+            assert pw not in seen, 'each (position, step) PTM drives one gate'
+            seen.add(pw)
+    meta = (tuple(shape), [tuple(int(q) for q in t) for t in targets],
+            [(int(a), int(c), [(int(p), int(w)) for p, w in gs]) for a, c, gs in segments],
+            max(1, int(checkpoint_every)))
+    return _Schedule.apply(x0, meta, *steps)
 
 
 _DIAG_TABLES = {}

@@ -18,14 +18,17 @@ into n // 2 dense blocks of up to three qubits
 over that prefix (64 x 64 PTMs on the MFMA kernel): about half the passes
 over the states of one launch per gate.  A circuit that ends inside a
 two-layer window gets identity PTMs for its missing layer; circuits that
-have finished are retired by narrowing the prefix, without copies.
+have finished are retired by narrowing the prefix, without copies.  A
+group's schedule is one autograd node (autograd.run_schedule), whose adjoint
+keeps the prefix's gradient in place and writes every block's dL/dP into
+one gradient tensor per block position.
 Forward, the float64 population readout and the adjoint
 (checkpointed recompute + qsv_ptm_grad) are all device work with no host
 synchronisation, so the whole step can be captured once in a HIP graph and
 replayed (``CircuitBatch.capture``).  The checkpoint interval is chosen
 from a memory budget: every intermediate state is kept when they fit (no
 recompute: on the 512-circuit batch the states of the wide circuits take
-~170 GB of the 288 GB HBM, and the step runs 0.31 s instead of 0.38 s with
+~175 GB of the 288 GB HBM, and the step runs 0.31 s instead of 0.38 s with
 a checkpoint every 16 blocks), else the shortest interval that fits.
 
 The reference simulates one circuit per PauliVector object; this module has
@@ -38,7 +41,7 @@ import numpy as np
 import torch
 
 from . import persistent
-from .autograd import Gate, run_circuit, diagonal
+from .autograd import Gate, run_schedule, diagonal
 from .autotune import tuning
 from .bases import general
 from .fusion import fuse_batched, fuse_plan
@@ -191,10 +194,10 @@ class CircuitBatch:
         return [t for g in self.groups for t in (g.angles, g.t1, g.t2)]
 
     def _layer_ptms(self, g):
-        """Fused PTMs of every layer of group g, computed for all layers of a
+        """PTMs of every layer of group g, computed for all layers of a
         parity in one batch: per parity p, a list over the layer's gates of
-        (per-layer [B, D, D] views, targets); layer l is entry l // 2 of
-        parity l % 2."""
+        ([layers of that parity, B, D, D] in the state's element type,
+        targets); layer l is step l // 2 of parity l % 2."""
         n, B = g.n, len(g.members)
         rot = rotation_ptms(g.angles)            # [B, L, n, 4, 4]
         damp = damping_ptms(g.t1, g.t2)          # [B, n, 4, 4]
@@ -206,8 +209,8 @@ class CircuitBatch:
                 continue
             rot_p = rot[:, p::2].reshape(B * Lp, n, 4, 4)
             damp_p = damp[:, None].expand(B, Lp, n, 4, 4).reshape(B * Lp, n, 4, 4)
-            # unbind: one view per layer, whose backward is a single stack
-            out.append([(gt.ptm.reshape(B, Lp, *gt.ptm.shape[1:]).unbind(1), gt.targets)
+            out.append([(gt.ptm.reshape(B, Lp, *gt.ptm.shape[1:]).transpose(0, 1)
+                         .to(self.dtype).contiguous(), gt.targets)
                         for gt in fused_layer_gates(n, p, rot_p, damp_p)])
         return out
 
@@ -261,15 +264,11 @@ class CircuitBatch:
             out.append((m.reshape(B, nw, D, D).transpose(0, 1).to(self.dtype).contiguous(), q))
         return out
 
-    def _window_gates(self, g, ptms, l, active):
-        """Gates of layer l for the `active` deepest circuits of group g
-        (one layer per window: no fusion)."""
-        return [Gate(P[l // 2][:active], t) for P, t in ptms[l % 2]]
-
     def _segments(self, g):
         """Runs of layers over which the running circuits of group g (a
-        prefix of its depth-sorted batch) do not change, one run_circuit
-        call each: [(first layer, end layer, running, still running after)]."""
+        prefix of its depth-sorted batch) do not change, one segment of
+        run_schedule each: [(first layer, end layer, running, still running
+        after)]."""
         deeper = lambda layer: sum(1 for d in g.layers if d > layer)
         out = []
         layer, active = 0, len(g.members)
@@ -295,56 +294,51 @@ class CircuitBatch:
         return sum(len(layer_targets(n, l % 2)) for l in range(first, end))
 
     def kept_bytes(self, k):
-        """Device bytes of the states the batched launches keep for the
-        adjoint with a checkpoint every k applications: per run_circuit call
-        its input and every k-th state, the k states one backward segment
-        recomputes, and the final states."""
+        """Device bytes the batched launches hold for the adjoint with a
+        checkpoint every k applications (an estimate of the step's peak):
+        each segment's input (the storage of the previous segment's output)
+        and every k-th state, the final states, and for the group whose
+        adjoint runs, its incoming gradient, two adjoint buffers and the
+        states one checkpoint interval recomputes."""
         es = torch.empty((), dtype=self.dtype).element_size()
-        total = seg = 0
+        total = extra = 0
         for gi, g in enumerate(self.groups):
             if gi in self.small:
                 continue
-            S = 4 ** g.n * es
-            total += len(g.members) * S
+            S, B = 4 ** g.n * es, len(g.members)
+            total += B * S
+            prev = B
             for first, end, active, _ in self._segments(g):
                 G = self._gates_in(g, first, end)
-                total += (1 + (G - 1) // k) * active * S
-                seg = max(seg, min(k, G) * active * S)
-        return total + seg
+                total += (prev + (G - 1) // k * active) * S
+                prev = active
+                extra = max(extra, ((min(k, G) - 1) * active + 3 * B) * S)
+        return total + extra
 
     def _group_populations(self, g):
         n, B = g.n, len(g.members)
-        N = 4 ** n
         if self.window == 2:
-            # one view per window (unbind: its backward is a single stack;
-            # slices of the stacked tensor would each backpropagate a
-            # zero-filled copy of all windows)
-            blocks = [(m.unbind(0), q) for m, q in self._window_blocks(g)]
+            blocks = self._window_blocks(g)
+            steps = [m for m, _ in blocks]
+            targets = [q for _, q in blocks]
+            step_of = lambda layer: [(p, layer // 2) for p in range(len(blocks))]
         else:
             ptms = self._layer_ptms(g)
-        x = torch.zeros((B, N), dtype=self.dtype, device=self.device)
+            steps = [P for par in ptms for P, _ in par]
+            targets = [t for par in ptms for _, t in par]
+            first = [0, len(ptms[0])]
+            step_of = lambda layer: [(first[layer % 2] + i, layer // 2)
+                                     for i in range(len(ptms[layer % 2]))]
+        # one segment per run of layers over which the running circuits (a
+        # prefix of the depth-sorted batch) do not change
+        segments = []
+        for lo, hi, active, nxt in self._segments(g):
+            gates = [pw for layer in range(lo, hi, self.window) for pw in step_of(layer)]
+            assert len(gates) == self._gates_in(g, lo, hi)
+            segments.append((active, nxt, gates))
+        x = torch.zeros((B, 4 ** n), dtype=self.dtype, device=self.device)
         x[:, 0] = 1
-        finals = [None] * B
-        for first, end, active, nxt in self._segments(g):
-            # windows while the active prefix (circuits deeper than the
-            # window's first layer) is unchanged: one checkpointed segment
-            gates = []
-            for layer in range(first, end, self.window):
-                if self.window == 2:
-                    w = layer // 2
-                    gates += [Gate(m[w][:active], q) for m, q in blocks]
-                else:
-                    gates += self._window_gates(g, ptms, layer, active)
-            assert len(gates) == self._gates_in(g, first, end)
-            x = run_circuit(x, gates, (4,) * n, self.k)
-            # one split into the continuing prefix and the finished states:
-            # its backward is one concatenation (separate slices would each
-            # backpropagate a zero-filled full-size gradient to be summed)
-            parts = torch.split(x, [nxt] + [1] * (active - nxt))
-            for b in range(nxt, active):
-                finals[b] = parts[1 + b - nxt]
-            x = parts[0]
-        states = torch.cat(finals, 0)
+        states = run_schedule(x, steps, targets, segments, (4,) * n, self.k)
         return diagonal(states, [general(2)] * n)
 
     def populations(self):

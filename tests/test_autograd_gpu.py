@@ -16,7 +16,7 @@ import torch
 
 import quantumsim_amd as qa
 from quantumsim_amd import bases
-from quantumsim_amd.autograd import apply, run_circuit, Gate, diagonal
+from quantumsim_amd.autograd import apply, run_circuit, run_schedule, Gate, diagonal
 from quantumsim_amd.workloads import TrainableCircuit, random_circuit_batch
 from quantumsim_amd.batch import CircuitBatch, plain_forward_backward
 from oracle.pauli_vector import PauliVectorNumpy
@@ -103,6 +103,46 @@ def test_checkpointed_circuit_equals_chain():
     for a, b in zip(g1, g2):
         assert torch.allclose(a, b, rtol=1e-12, atol=1e-12)
 
+
+
+@pytest.mark.parametrize('k', [1, 2, 3])
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12), (torch.float32, 2e-5)])
+def test_run_schedule_matches_per_state_chains(k, dtype, tol):
+    """A shrinking schedule (5 states, then 3, then 1; positions of 1, 2
+    and 3 qubits, the last on the 64 x 64 MFMA kernels) against each state
+    run alone through the gates of the segments it takes part in, values
+    and gradients of x0 and of every step tensor."""
+    rng = np.random.default_rng(5)
+    n, B = 5, 5
+    shape = (4,) * n
+    positions = [(0, 1), (4,), (1, 2, 3)]
+    steps = [torch.tensor(rng.standard_normal((3, B, 4 ** len(t), 4 ** len(t))) / 2 ** len(t),
+                          dtype=dtype, device=DEV, requires_grad=True) for t in positions]
+    segments = [(5, 3, [(0, 0), (1, 0), (2, 0), (0, 1), (1, 1)]),
+                (3, 1, [(2, 1), (0, 2)]),
+                (1, 0, [(1, 2), (2, 2)])]
+    x0 = torch.tensor(rng.standard_normal((B, 4 ** n)), dtype=dtype, device=DEV,
+                      requires_grad=True)
+    w = torch.tensor(rng.standard_normal((B, 4 ** n)), dtype=dtype, device=DEV)
+    out = run_schedule(x0, steps, positions, segments, shape, k)
+    g1 = torch.autograd.grad((out * w).sum(), [x0] + steps)
+    rows = []
+    for b in range(B):
+        y = x0[b:b + 1]
+        for active, _, gates in segments:
+            if b < active:
+                for p, s in gates:
+                    y = apply(y, steps[p][s][b:b + 1], shape, positions[p])
+        rows.append(y)
+    ref = torch.cat(rows)
+    g2 = torch.autograd.grad((ref * w).sum(), [x0] + steps)
+    scale = lambda t: max(1.0, t.abs().max().item())
+    assert (out - ref).abs().max().item() <= tol * scale(ref)
+    for a, b in zip(g1, g2):
+        assert (a - b).abs().max().item() <= tol * scale(b)
+    # PTMs a state never meets get a zero gradient
+    for grad, rows_unused in ((g1[1][2], 3), (g1[2][2], 1), (g1[3][2], 1), (g1[3][1], 3)):
+        assert grad[rows_unused:].abs().max().item() == 0
 
 def _kl_loss(tc, params, target, dtype):
     ang, t1, t2 = params
