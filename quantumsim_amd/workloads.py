@@ -199,13 +199,45 @@ def fused_layer_gates(n, layer, rot, damp):
     return gates
 
 
+_ROT_TERMS = {}
+
+
+def _rotation_terms(device):
+    """M [9, 16] with PTM(RZ(b) RX(a)) = sum over i, j of f_j(b) f_i(a)
+    M[3 j + i] for f = (1, cos, sin).  A one-qubit rotation's PTM is a
+    first-order trigonometric polynomial of its angle (its Kraus operator's
+    entries are cos(t/2), sin(t/2) or e^(+-it/2), and the PTM is quadratic
+    in them), so P(t) = A + B cos t + C sin t with A = (P(0) + P(pi)) / 2,
+    B = (P(0) - P(pi)) / 2, C = P(pi/2) - A, read off the Kraus-built PTMs
+    (autograd.rx_ptm / rz_ptm) once per device."""
+    M = _ROT_TERMS.get(device)
+    if M is None:
+        import torch
+        from .autograd import rx_ptm, rz_ptm
+        t = torch.tensor([0.0, np.pi, np.pi / 2], dtype=torch.float64, device=device)
+
+        def terms(P):
+            A = (P[0] + P[1]) / 2
+            return [A, (P[0] - P[1]) / 2, P[2] - A]
+        X, Z = terms(rx_ptm(t)), terms(rz_ptm(t))
+        M = torch.stack([(Z[j] @ X[i]).reshape(16) for j in range(3) for i in range(3)])
+        _ROT_TERMS[device] = M
+    return M
+
+
 def rotation_ptms(angles):
-    """angles [..., 2] (RX angle, RZ angle) -> PTMs [..., 4, 4] of RZ RX."""
-    from .autograd import rx_ptm, rz_ptm
+    """angles [..., 2] (RX angle, RZ angle) -> PTMs [..., 4, 4] of RZ RX,
+    as one real product with the nine trigonometric terms (no complex
+    Kraus arithmetic per angle; equal to rz_ptm(b) @ rx_ptm(a) to rounding)."""
+    import torch
     shp = angles.shape[:-1]
     a = angles.reshape(-1, 2)
-    p = rz_ptm(a[:, 1]) @ rx_ptm(a[:, 0])
-    return p.reshape(*shp, 4, 4)
+    one = torch.ones_like(a[:, 0])
+    fx = torch.stack([one, torch.cos(a[:, 0]), torch.sin(a[:, 0])], -1)
+    fz = torch.stack([one, torch.cos(a[:, 1]), torch.sin(a[:, 1])], -1)
+    coef = (fz[:, :, None] * fx[:, None, :]).reshape(-1, 9)
+    M = _rotation_terms(angles.device).to(angles.dtype)
+    return (coef @ M).reshape(*shp, 4, 4)
 
 
 def damping_ptms(t1, t2, t=GATE_TIME * 1e6):

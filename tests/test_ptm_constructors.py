@@ -1,0 +1,76 @@
+"""PTM-from-parameter constructors of the differentiable path, on the CPU
+(torch ops only): the Kraus-built RX / RZ / damping PTMs against the
+oracle's kraus_to_ptm (quantumsim/algebra/algebra.py:11-30) on the same
+Kraus sets, and the closed-form rotation PTMs the batched and trainable
+circuits build against the Kraus route, with their gradients."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import algebra as oa, bases as ob
+from quantumsim_amd.autograd import rx_ptm, rz_ptm, damping_ptm
+from quantumsim_amd.workloads import rotation_ptms
+
+QB = [ob.general(2)]
+
+
+def _rx(t):
+    c, s = np.cos(t / 2), np.sin(t / 2)
+    return np.array([[c, -1j * s], [-1j * s, c]])
+
+
+def _rz(t):
+    return np.diag([np.exp(-0.5j * t), np.exp(0.5j * t)])
+
+
+@pytest.mark.parametrize('theta', [0.0, 0.3, np.pi / 2, np.pi, -2.1, 5.9])
+def test_rotation_constructors_match_oracle(theta):
+    t = torch.tensor([theta], dtype=torch.float64)
+    for make, kraus in ((rx_ptm, _rx), (rz_ptm, _rz)):
+        got = make(t)[0].numpy()
+        want = oa.kraus_to_ptm(kraus(theta)[None], QB, QB)
+        np.testing.assert_allclose(got, want, atol=1e-14, rtol=0)
+
+
+@pytest.mark.parametrize('t1,t2', [(30.0, 20.0), (25.0, 50.0), (40.0, 12.0)])
+def test_damping_constructor_matches_oracle(t1, t2):
+    """Amplitude damping (gamma = 1 - e^(-t/T1)) then phase damping
+    (lambda = 1 - e^(-2t/T_phi), 1/T_phi = 1/T2 - 1/(2 T1))."""
+    t = 0.02
+    g = 1 - np.exp(-t / t1)
+    lam = 1 - np.exp(-2 * t * (1 / t2 - 0.5 / t1))
+    ka = np.array([[[1, 0], [0, np.sqrt(1 - g)]], [[0, np.sqrt(g)], [0, 0]]])
+    kp = np.array([[[1, 0], [0, np.sqrt(1 - lam)]], [[0, 0], [0, np.sqrt(lam)]]])
+    want = oa.kraus_to_ptm(kp, QB, QB) @ oa.kraus_to_ptm(ka, QB, QB)
+    got = damping_ptm(torch.tensor([t1], dtype=torch.float64),
+                      torch.tensor([t2], dtype=torch.float64), t)[0].numpy()
+    np.testing.assert_allclose(got, want, atol=1e-14, rtol=0)
+    # coherences decay as e^(-t/T2), populations relax as e^(-t/T1)
+    assert np.isclose(got[2, 2], np.exp(-t / t2)) and np.isclose(got[3, 3], np.exp(-t / t2))
+    assert np.isclose(got[0, 1], 1 - np.exp(-t / t1))
+
+
+def test_rotation_ptms_closed_form_equals_kraus_route():
+    rng = np.random.default_rng(0)
+    a = torch.tensor(rng.uniform(-7, 7, size=(40, 3, 2)), dtype=torch.float64)
+    got = rotation_ptms(a)
+    flat = a.reshape(-1, 2)
+    want = (rz_ptm(flat[:, 1]) @ rx_ptm(flat[:, 0])).reshape(40, 3, 4, 4)
+    assert (got - want).abs().max().item() < 1e-14
+    # and against the oracle, entry by entry
+    for (x, z), p in zip(flat.numpy()[:6], got.reshape(-1, 4, 4)[:6].numpy()):
+        ref = oa.kraus_to_ptm(_rz(z)[None], QB, QB) @ oa.kraus_to_ptm(_rx(x)[None], QB, QB)
+        np.testing.assert_allclose(p, ref, atol=1e-14, rtol=0)
+    # float32 angles give float32 PTMs of the same values
+    got32 = rotation_ptms(a.float())
+    assert got32.dtype == torch.float32 and (got32.double() - want).abs().max().item() < 1e-6
+
+
+def test_rotation_ptms_gradient():
+    rng = np.random.default_rng(1)
+    a = torch.tensor(rng.uniform(-3, 3, size=(5, 2)), dtype=torch.float64, requires_grad=True)
+    assert torch.autograd.gradcheck(rotation_ptms, (a,), eps=1e-6, atol=1e-8)
+    w = torch.tensor(rng.standard_normal((5, 4, 4)), dtype=torch.float64)
+    g1 = torch.autograd.grad((rotation_ptms(a) * w).sum(), a)[0]
+    g2 = torch.autograd.grad(((rz_ptm(a[:, 1]) @ rx_ptm(a[:, 0])) * w).sum(), a)[0]
+    assert (g1 - g2).abs().max().item() < 1e-13
