@@ -328,11 +328,34 @@ def run_schedule(x0, steps, targets, segments, shape, checkpoint_every=1):
     per position, so it makes no per-gate slice gradients, stacks or
     concatenations.
     """
+    B = x0.shape[0]
+    if x0.dim() != 2 or x0.shape[1] != math.prod(shape) or not x0.is_contiguous():
+        raise ValueError('x0 must be contiguous [B, {}]'.format(math.prod(shape)))
+    if len(steps) != len(targets):
+        raise ValueError('one targets tuple per step position')
+    for t, tg in zip(steps, targets):
+        D = math.prod(shape[q] for q in tg)
+        if (t.dim() != 4 or t.shape[1] != B or tuple(t.shape[2:]) != (D, D)
+                or t.dtype != x0.dtype or not t.is_contiguous()):
+            raise ValueError('steps tensors must be contiguous [n_steps, {}, D, D] of the '
+                             'states\' type'.format(B))
+    # the running prefix starts with every state, shrinks from segment to
+    # segment and ends empty, so every state's final value is written once
+    prev = B
     seen = set()
-    for _, _, gates in segments:
-        for pw in gates:
-            assert pw not in seen, 'each (position, step) PTM drives one gate'
-            seen.add(pw)
+    for active, cont, gates in segments:
+        if active != prev or not 0 <= cont <= active:
+            raise ValueError('segments must start with all {} states and hand each '
+                             'segment the previous one\'s continuing states'.format(B))
+        prev = cont
+        for p, w in gates:
+            if not (0 <= p < len(steps) and 0 <= w < steps[p].shape[0]):
+                raise ValueError('gate ({}, {}) outside the steps tensors'.format(p, w))
+            if (p, w) in seen:
+                raise ValueError('each (position, step) PTM drives one gate')
+            seen.add((p, w))
+    if prev != 0:
+        raise ValueError('the last segment must leave no state running')
     meta = (tuple(shape), [tuple(int(q) for q in t) for t in targets],
             [(int(a), int(c), [(int(p), int(w)) for p, w in gs]) for a, c, gs in segments],
             max(1, int(checkpoint_every)))

@@ -91,3 +91,33 @@ def test_choose_checkpoint_interval():
     assert choose_checkpoint_interval(cb.kept_bytes, k4) == 4
     assert choose_checkpoint_interval(cb.kept_bytes, 0) == 16
     assert choose_checkpoint_interval(lambda k: 100 // k, 30, (1, 3, 5)) == 5
+
+
+def test_run_schedule_rejects_inconsistent_arguments():
+    """Argument checks run on the host before any launch."""
+    from quantumsim_amd.autograd import run_schedule
+    shape = (4, 4, 4)
+    x0 = torch.zeros((3, 64))
+    steps = [torch.zeros((2, 3, 16, 16)), torch.zeros((2, 3, 4, 4))]
+    targets = [(0, 1), (2,)]
+    ok = [(3, 1, [(0, 0), (1, 0)]), (1, 0, [(0, 1), (1, 1)])]
+    bad = {
+        'prefix does not start with every state': [(2, 0, [(0, 0)])],
+        'segments do not chain': [(3, 1, [(0, 0)]), (2, 0, [(0, 1)])],
+        'states left running': [(3, 1, [(0, 0)])],
+        'continuing above running': [(3, 4, [(0, 0)])],
+        'PTM reused': [(3, 0, [(0, 0), (0, 0)])],
+        'step out of range': [(3, 0, [(0, 2)])],
+        'position out of range': [(3, 0, [(2, 0)])],
+    }
+    for what, segs in bad.items():
+        with pytest.raises(ValueError):
+            run_schedule(x0, steps, targets, segs, shape)
+    with pytest.raises(ValueError):          # wrong element type
+        run_schedule(x0, [s.double() for s in steps], targets, ok, shape)
+    with pytest.raises(ValueError):          # wrong PTM size for the targets
+        run_schedule(x0, steps, [(0,), (2,)], ok, shape)
+    with pytest.raises(ValueError):          # state size does not match the shape
+        run_schedule(torch.zeros((3, 65)), steps, targets, ok, shape)
+    with pytest.raises(ValueError):          # batch size differs
+        run_schedule(torch.zeros((2, 64)), steps, targets, ok, shape)
