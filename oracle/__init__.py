@@ -3,7 +3,7 @@
 A clean-room numpy float64 restatement of quantumsim's reference backend
 (quantumsim/pauli_vectors/numpy.py, pauli_vector.py, bases/, algebra/).  Only
 tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
-it, and only as the checker or the CPU baseline; the product
+it, and only to check results or as the CPU baseline; the product
 (quantumsim_amd) never does.
 
 Pinning (what checks each function, see tests/test_oracle.py):

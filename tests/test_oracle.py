@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/) before the tests trust it as their reference.
 
 Each check is chosen so a plausible mistake (wrong element order, a dropped
 conjugate, transposed PTM, mislabelled axis) fails it: the reference's own
