@@ -174,6 +174,7 @@ def bench_batched(args):
     # probability error of the float32 step against the same batch in float64
     cb.replay()
     torch.cuda.synchronize()
+    trace_dev = max((t - 1).abs().max().item() for t in cb.g_traces)
     cb64 = CircuitBatch(circuits, dtype=torch.float64, checkpoint_every=16,
                         fuse_qudits=args.fuse_qudits)
     with torch.no_grad():
@@ -192,6 +193,8 @@ def bench_batched(args):
            'peak_alloc_gb': round(peak_gb, 1), 'free_gb_at_start': round(free0 / 1e9, 1),
            'setup_s': round(t_setup, 1),
            'max_prob_err_vs_f64': float('%.3e' % prob_err), 'fuse_qudits': args.fuse_qudits,
+           'readout': 'float64 populations / float64 trace per circuit',
+           'max_trace_dev_f32': float('%.3e' % trace_dev),
            'autotune': tuner.summary()}
     log('batched: %s' % res)
     del cb

@@ -85,14 +85,17 @@ class CircuitBatch:
 
     Parameters are the leaf tensors ``group.angles`` [B, L, n, 2] and
     ``group.t1``/``group.t2`` [B, n] (microseconds) of each group; the step
-    returns float64 populations per circuit and, with a loss, their
+    returns float64 populations per circuit (divided by the circuit's
+    float64 trace unless ``renormalize=False``) and, with a loss, their
     gradients with respect to every parameter."""
 
     def __init__(self, circuits, dtype=torch.float32, device=None, checkpoint_every=None,
                  requires_grad=True, persistent_kernel=True, fuse_qudits=3,
-                 checkpoint_budget=None):
+                 checkpoint_budget=None, renormalize=True):
         self.device = device or torch.device('cuda', torch.cuda.current_device())
         self.dtype = dtype
+        self.renormalize = renormalize
+        self.traces = None
         # circuits in batched launches: layers taken two at a time and fused
         # into dense blocks of up to fuse_qudits qubits (3: 64 x 64 PTMs on
         # the MFMA kernels; 2 or less: the layer's own gates, one at a time)
@@ -342,7 +345,15 @@ class CircuitBatch:
         return diagonal(states, [general(2)] * n)
 
     def populations(self):
-        """float64 populations per group ([B_n, 2^n] tensors, depth-sorted)."""
+        """float64 populations per group ([B_n, 2^n] tensors, depth-sorted).
+
+        With ``renormalize`` (the default) each circuit's populations are
+        divided by their float64 sum, the state's trace, when it exceeds
+        1e-8 (quantumsim/pauli_vectors/numpy.py:109-117, without the
+        warning, which would need a host synchronisation): the float32
+        states' rounding drift of the trace is removed in float64.  The
+        traces before normalisation ([B_n] float64 per group, what the
+        reference's renormalize returns) are kept in ``self.traces``."""
         out = [None] * len(self.groups)
         if self.small:
             for gi, p in zip(self.small, self._persistent_populations()):
@@ -350,6 +361,10 @@ class CircuitBatch:
         for gi, g in enumerate(self.groups):
             if out[gi] is None:
                 out[gi] = self._group_populations(g)
+        self.traces = [p.detach().sum(-1) for p in out]
+        if self.renormalize:
+            out = [torch.where(t[:, None] > 1e-8, p / p.sum(-1, keepdim=True), p)
+                   for p, t in zip(out, self.traces)]
         return out
 
     def _step(self, weights):
@@ -391,6 +406,7 @@ class CircuitBatch:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph, stream=self.stream):
             self.g_loss, self.g_pops, self.g_grads = self._step(weights)
+            self.g_traces = self.traces
         torch.cuda.synchronize(self.device)
         return self.graph
 

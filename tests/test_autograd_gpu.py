@@ -477,3 +477,27 @@ def test_circuit_batch_checkpoint_interval_from_budget():
             assert torch.equal(a, b)
         for a, b in zip(gr, g1):
             assert torch.allclose(a, b, rtol=1e-12, atol=1e-14)
+
+
+def test_circuit_batch_fp64_trace_and_renormalize():
+    """Float64 traces of every circuit (the sum of its float64 populations,
+    what the reference's renormalize returns) and populations divided by
+    them; the gradient goes through the normalisation."""
+    circuits = random_circuit_batch(10, seed=31, n_range=(4, 9), layer_range=(3, 9))
+    for dtype, tol in ((torch.float64, 1e-13), (torch.float32, 2e-6)):
+        cb = CircuitBatch(circuits, dtype=dtype)
+        raw = CircuitBatch(circuits, dtype=dtype, renormalize=False)
+        w = cb.random_weights(5)
+        _, p, g = cb.step(w)
+        q = raw.populations()
+        for a, b, t in zip(p, q, cb.traces):
+            assert t.dtype == torch.float64 and t.shape == (a.shape[0],)
+            assert torch.allclose(t, b.detach().sum(-1), rtol=0, atol=1e-15)
+            # trace-preserving channels: only the float32 states' rounding moves it
+            assert (t - 1).abs().max().item() < (1e-13 if dtype == torch.float64 else 1e-5)
+            assert (a.detach().sum(-1) - 1).abs().max().item() < 1e-14
+            assert torch.allclose(a.detach(), b.detach() / t[:, None], rtol=1e-14, atol=0)
+        loss = sum(((b / b.sum(-1, keepdim=True)) * wt).sum() for b, wt in zip(q, w))
+        g2 = torch.autograd.grad(loss, raw.parameters())
+        for a, b in zip(g, g2):
+            assert (a - b).abs().max().item() <= tol * max(1.0, b.abs().max().item())
