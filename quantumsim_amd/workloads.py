@@ -186,16 +186,19 @@ def fused_layer_gates(n, layer, rot, damp):
     """Gates of one layer for a batch: rot, damp [B, n, 4, 4] PTMs."""
     from .autograd import Gate, cz_ptm, kron_ptm
     cz = cz_ptm(rot.device).to(rot.dtype)
+    # one view per qubit (unbind: a single stack in the backward instead of
+    # a zero-filled full-size gradient per selected qubit)
+    rq, dq = rot.unbind(1), damp.unbind(1)
     gates = []
     for t in layer_targets(n, layer):
         if len(t) == 2:
             q, q2 = t
-            pre = kron_ptm(rot[:, q], rot[:, q2])
-            post = kron_ptm(damp[:, q], damp[:, q2])
+            pre = kron_ptm(rq[q], rq[q2])
+            post = kron_ptm(dq[q], dq[q2])
             gates.append(Gate(post @ cz @ pre, t))
         else:
             q = t[0]
-            gates.append(Gate(damp[:, q] @ rot[:, q], t))
+            gates.append(Gate(dq[q] @ rq[q], t))
     return gates
 
 
