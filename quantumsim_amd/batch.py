@@ -91,7 +91,7 @@ class CircuitBatch:
 
     def __init__(self, circuits, dtype=torch.float32, device=None, checkpoint_every=None,
                  requires_grad=True, persistent_kernel=True, fuse_qudits=3,
-                 checkpoint_budget=None, renormalize=True):
+                 checkpoint_budget=None, renormalize=True, overlap_narrow=True):
         self.device = device or torch.device('cuda', torch.cuda.current_device())
         self.dtype = dtype
         self.renormalize = renormalize
@@ -107,6 +107,10 @@ class CircuitBatch:
         # captured graph, so autograd never crosses streams
         self.stream = torch.cuda.Stream(device=self.device)
         self.stream.wait_stream(torch.cuda.current_stream(self.device))
+        # the narrow circuits' persistent kernels (one workgroup per CU,
+        # latency-bound) run on a second stream, also of default priority,
+        # beside the wide groups' bandwidth-bound launches
+        self.side = torch.cuda.Stream(device=self.device) if overlap_narrow else None
         by_n = {}
         for i, c in enumerate(circuits):
             by_n.setdefault(c['n'], []).append(i)
@@ -355,12 +359,23 @@ class CircuitBatch:
         traces before normalisation ([B_n] float64 per group, what the
         reference's renormalize returns) are kept in ``self.traces``."""
         out = [None] * len(self.groups)
-        if self.small:
-            for gi, p in zip(self.small, self._persistent_populations()):
-                out[gi] = p
+        cur = torch.cuda.current_stream(self.device)
+        side = self.side if self.small else None
+        if side is not None:
+            side.wait_stream(cur)
+            with torch.cuda.stream(side):
+                small = self._persistent_populations()
+        elif self.small:
+            small = self._persistent_populations()
         for gi, g in enumerate(self.groups):
-            if out[gi] is None:
+            if gi not in self.small:
                 out[gi] = self._group_populations(g)
+        if side is not None:
+            cur.wait_stream(side)
+            for p in small:
+                p.record_stream(cur)
+        for gi, p in zip(self.small, small if self.small else []):
+            out[gi] = p
         self.traces = [p.detach().sum(-1) for p in out]
         if self.renormalize:
             out = [torch.where(t[:, None] > 1e-8, p / p.sum(-1, keepdim=True), p)
@@ -371,6 +386,13 @@ class CircuitBatch:
         pops = self.populations()
         loss = sum((p * w).sum() for p, w in zip(pops, weights))
         grads = torch.autograd.grad(loss, self.parameters())
+        if self.side is not None and self.small:
+            # the narrow groups' adjoint ran on the side stream: join it
+            cur = torch.cuda.current_stream(self.device)
+            cur.wait_stream(self.side)
+            for gi in self.small:
+                for t in grads[3 * gi:3 * gi + 3]:
+                    t.record_stream(cur)
         return loss, pops, grads
 
     def step(self, weights):

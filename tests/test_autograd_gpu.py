@@ -501,3 +501,27 @@ def test_circuit_batch_fp64_trace_and_renormalize():
         g2 = torch.autograd.grad(loss, raw.parameters())
         for a, b in zip(g, g2):
             assert (a - b).abs().max().item() <= tol * max(1.0, b.abs().max().item())
+
+
+def test_circuit_batch_side_stream_equals_one_stream():
+    """The narrow circuits' persistent kernels on a second stream give the
+    one-stream results, eagerly and in graph replays (joined streams)."""
+    circuits = random_circuit_batch(14, seed=41, n_range=(4, 9), layer_range=(3, 12))
+    one = CircuitBatch(circuits, dtype=torch.float32, overlap_narrow=False)
+    two = CircuitBatch(circuits, dtype=torch.float32)
+    assert one.side is None and two.side is not None and two.small
+    w = one.random_weights(4)
+    _, p1, g1 = one.step(w)
+    _, p2, g2 = two.step(w)
+    for a, b in zip(p1, p2):
+        assert torch.equal(a, b)
+    for a, b in zip(g1, g2):
+        assert torch.equal(a, b)
+    two.capture(w, autotune=False)
+    for _ in range(3):
+        _, pr, gr = two.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(pr, p1):
+            assert torch.equal(a, b)
+        for a, b in zip(gr, g1):
+            assert torch.equal(a, b)
