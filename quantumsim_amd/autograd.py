@@ -40,14 +40,19 @@ def _stream():
 
 
 class _Workspace:
+    """Scratch of the PTM-gradient partial sums, one buffer per (device,
+    stream): launches on one stream run in order and may share it, launches
+    on two streams may overlap and must not.  Grown outside graph capture
+    (the eager warm-up sizes it)."""
     _bufs = {}
 
     @classmethod
     def get(cls, device, nbytes):
-        t = cls._bufs.get(device)
+        key = (device, torch.cuda.current_stream(device).cuda_stream)
+        t = cls._bufs.get(key)
         if t is None or t.numel() < nbytes:
             t = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
-            cls._bufs[device] = t
+            cls._bufs[key] = t
         return t
 
 
@@ -435,7 +440,9 @@ def diagonal(x, bases):
             offsets.append(len(flat_all))
             flat_all.extend(t)
         idx_dev = torch.tensor(flat_all, dtype=torch.int32, device=x.device)
-        hit = (offsets, idx_dev, _scatter_index(shape, take, x.device))
+        src, pos = _scatter_index(shape, take, x.device)
+        # tables are shared by every caller of this geometry, on any stream
+        hit = (offsets, shared_constant(idx_dev), (src, shared_constant(pos)))
         _DIAG_TABLES[key] = hit
     offsets, idx_dev, scatter = hit
     return _Diagonal.apply(x.contiguous(), shape, take, offsets, [len(t) for t in take],
@@ -446,13 +453,22 @@ def diagonal(x, bases):
 _BASIS_CACHE = {}
 
 
+def shared_constant(t):
+    """Make a freshly built device constant safe to read from any stream:
+    wait until the stream that built it has finished (once per constant;
+    never during graph capture, where the eager warm-up has built it)."""
+    if t.is_cuda and not torch.cuda.is_current_stream_capturing():
+        torch.cuda.current_stream(t.device).synchronize()
+    return t
+
+
 def qubit_basis_tensor(device, dtype=torch.complex128):
     """general(2) basis matrices as a [4, 2, 2] complex tensor."""
     key = (device, dtype)
     t = _BASIS_CACHE.get(key)
     if t is None:
         t = torch.tensor(np.asarray(general(2).vectors), dtype=dtype, device=device)
-        _BASIS_CACHE[key] = t
+        _BASIS_CACHE[key] = shared_constant(t)
     return t
 
 
@@ -508,7 +524,7 @@ def cz_ptm(device):
         from .operations.qubits import cphase
         t = torch.tensor(cphase(np.pi).ptm.reshape(16, 16), dtype=torch.float64,
                          device=device).unsqueeze(0)
-        _BASIS_CACHE[key] = t
+        _BASIS_CACHE[key] = shared_constant(t)
     return t
 
 

@@ -41,7 +41,7 @@ import numpy as np
 import torch
 
 from . import persistent
-from .autograd import Gate, run_schedule, diagonal
+from .autograd import Gate, run_schedule, diagonal, shared_constant
 from .autotune import tuning
 from .bases import general
 from .fusion import fuse_batched, fuse_plan
@@ -104,12 +104,15 @@ class CircuitBatch:
         self._consts = {}
         self.circuits = circuits
         # one stream (default priority) for parameters, eager steps and the
-        # captured graph, so autograd never crosses streams
+        # captured graph
         self.stream = torch.cuda.Stream(device=self.device)
         self.stream.wait_stream(torch.cuda.current_stream(self.device))
         # the narrow circuits' persistent kernels (one workgroup per CU,
-        # latency-bound) run on a second stream, also of default priority,
-        # beside the wide groups' bandwidth-bound launches
+        # latency-bound) and part of the batched groups run on a second
+        # stream, also of default priority, beside the other groups'
+        # launches (filling each other's launch gaps and kernel tails);
+        # autograd runs each group's adjoint on the stream of its forward,
+        # and the step joins the two
         self.side = torch.cuda.Stream(device=self.device) if overlap_narrow else None
         by_n = {}
         for i, c in enumerate(circuits):
@@ -126,6 +129,11 @@ class CircuitBatch:
         # narrow circuits: one persistent launch (tables built once here)
         self.lds_qubits = persistent.max_qubits(dtype) if persistent_kernel else 0
         self.small = [gi for gi, g in enumerate(self.groups) if g.n <= self.lds_qubits]
+        # batched groups split over the two streams by their state traffic
+        # (largest first, each to the stream with less so far; the first
+        # stays on the main stream): on the benchmark batch the 11-qubit
+        # group runs beside the 8-10-qubit ones
+        self.side_groups = self._balance_streams() if overlap_narrow else []
         self.ptable = self._persistent_table() if self.small else None
         # checkpoint interval of the batched launches: None = the shortest
         # of CHECKPOINT_INTERVALS whose kept states fit checkpoint_budget
@@ -223,10 +231,11 @@ class CircuitBatch:
 
     def _cached(self, key, make):
         """Constant device tensors (masks, identities) made once, in an eager
-        step, so a captured graph never copies from the host."""
+        step, so a captured graph never copies from the host; readable from
+        both streams once made."""
         t = self._consts.get(key)
         if t is None:
-            t = make()
+            t = shared_constant(make())
             self._consts[key] = t
         return t
 
@@ -360,22 +369,24 @@ class CircuitBatch:
         reference's renormalize returns) are kept in ``self.traces``."""
         out = [None] * len(self.groups)
         cur = torch.cuda.current_stream(self.device)
-        side = self.side if self.small else None
+        side = self.side if self._side_groups() else None
         if side is not None:
             side.wait_stream(cur)
             with torch.cuda.stream(side):
-                small = self._persistent_populations()
-        elif self.small:
-            small = self._persistent_populations()
+                small = self._persistent_populations() if self.small else []
+                for gi in self.side_groups:
+                    out[gi] = self._group_populations(self.groups[gi])
+        else:
+            small = self._persistent_populations() if self.small else []
         for gi, g in enumerate(self.groups):
-            if gi not in self.small:
+            if gi not in self.small and out[gi] is None:
                 out[gi] = self._group_populations(g)
+        for gi, p in zip(self.small, small):
+            out[gi] = p
         if side is not None:
             cur.wait_stream(side)
-            for p in small:
-                p.record_stream(cur)
-        for gi, p in zip(self.small, small if self.small else []):
-            out[gi] = p
+            for gi in self._side_groups():
+                out[gi].record_stream(cur)
         self.traces = [p.detach().sum(-1) for p in out]
         if self.renormalize:
             out = [torch.where(t[:, None] > 1e-8, p / p.sum(-1, keepdim=True), p)
@@ -386,14 +397,33 @@ class CircuitBatch:
         pops = self.populations()
         loss = sum((p * w).sum() for p, w in zip(pops, weights))
         grads = torch.autograd.grad(loss, self.parameters())
-        if self.side is not None and self.small:
-            # the narrow groups' adjoint ran on the side stream: join it
+        if self.side is not None and self._side_groups():
+            # the side stream's groups ran their adjoint there: join it
             cur = torch.cuda.current_stream(self.device)
             cur.wait_stream(self.side)
-            for gi in self.small:
+            for gi in self._side_groups():
                 for t in grads[3 * gi:3 * gi + 3]:
                     t.record_stream(cur)
         return loss, pops, grads
+
+    def _balance_streams(self):
+        """Batched groups for the side stream: longest-processing-time
+        assignment of the groups' state traffic (sum over circuits of
+        windows x 4^n) to the two streams, the largest on the main one."""
+        cost = {gi: sum((d + 1) // 2 for d in g.layers) * 4 ** g.n
+                for gi, g in enumerate(self.groups) if gi not in self.small}
+        load, side = [0, 0], []
+        for gi in sorted(cost, key=lambda gi: -cost[gi]):
+            s = 0 if load[0] <= load[1] else 1
+            load[s] += cost[gi]
+            if s:
+                side.append(gi)
+        return sorted(side)
+
+    def _side_groups(self):
+        """Groups whose work runs on the side stream: the persistent ones
+        and the batched ones of small states (launch-bound)."""
+        return (list(self.small) + list(self.side_groups)) if self.side is not None else []
 
     def step(self, weights):
         """Forward + adjoint of loss = sum over circuits of <w_c, p_c>;

@@ -175,7 +175,9 @@ def embed_ptm_batched(ptm, qudits, block, dims):
             key = (dims[block[b]], ptm.dtype, ptm.device)
             eye = _EYES.get(key)
             if eye is None:
-                eye = _EYES[key] = torch.eye(key[0], dtype=ptm.dtype, device=ptm.device)
+                from .autograd import shared_constant
+                eye = _EYES[key] = shared_constant(
+                    torch.eye(key[0], dtype=ptm.dtype, device=ptm.device))
             ops += [eye, [1 + b, 1 + m + b]]
     out = torch.einsum(*ops, list(range(2 * m + 1)))
     D = int(np.prod([dims[q] for q in block]))

@@ -224,7 +224,8 @@ def _rotation_terms(device):
             return [A, (P[0] - P[1]) / 2, P[2] - A]
         X, Z = terms(rx_ptm(t)), terms(rz_ptm(t))
         M = torch.stack([(Z[j] @ X[i]).reshape(16) for j in range(3) for i in range(3)])
-        _ROT_TERMS[device] = M
+        from .autograd import shared_constant
+        _ROT_TERMS[device] = shared_constant(M)
     return M
 
 

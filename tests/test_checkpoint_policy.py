@@ -121,3 +121,18 @@ def test_run_schedule_rejects_inconsistent_arguments():
         run_schedule(torch.zeros((3, 65)), steps, targets, ok, shape)
     with pytest.raises(ValueError):          # batch size differs
         run_schedule(torch.zeros((2, 64)), steps, targets, ok, shape)
+
+
+def test_stream_balance():
+    """Batched groups go to the stream with less state traffic so far,
+    largest first; the largest stays on the main stream."""
+    cb = _batch({8: [40] * 60, 9: [40] * 60, 10: [40] * 60, 11: [40] * 60})
+    side = [cb.groups[g].n for g in cb._balance_streams()]
+    assert side == [8, 9, 10]                  # 11 qubits: 4x the 10-qubit traffic
+    cb = _batch({10: [40] * 10, 11: [40] * 10})
+    assert [cb.groups[g].n for g in cb._balance_streams()] == [10]
+    cb = _batch({11: [40] * 10})
+    assert cb._balance_streams() == []           # one group: nothing beside it
+    cb = _batch({9: [10] * 4, 10: [10] * 4})
+    cb.small = [0]                               # persistent groups are not balanced
+    assert [cb.groups[g].n for g in cb._balance_streams()] == []
