@@ -308,7 +308,7 @@ class _Schedule(torch.autograd.Function):
                     _apply_raw(gy, steps_t[pos][w], shape, targets[pos], dims[pos],
                                out=bufs[1 - cur][:active])
                     cur = 1 - cur
-        gx = bufs[cur].clone() if ctx.needs_input_grad[0] else None
+        gx = bufs[cur] if ctx.needs_input_grad[0] else None
         return (gx, None, *grads)
 
 
