@@ -243,7 +243,8 @@ def bench_circuit(args, circuit, n):
 
 def bench_differentiable(args):
     """12 qubits, 40 layers, 96 parameters x 64 parameter sets, float32
-    states: KL loss forward + adjoint (checkpoint every 8 gates)."""
+    states: KL loss forward + adjoint (checkpoint every 8 gates), timed
+    eagerly and as replays of the step captured in a HIP graph."""
     import torch
     from quantumsim_amd.workloads import TrainableCircuit
     tc = TrainableCircuit(n=12, layers=40)
@@ -262,13 +263,35 @@ def bench_differentiable(args):
 
     step()
     torch.cuda.synchronize()
-    sec = timed(step, 2, torch.cuda.current_stream())
+    sec_eager = timed(step, 2, torch.cuda.current_stream())
+    # the same step captured once in a HIP graph (on a stream of its own,
+    # default priority) and replayed: no host launch gaps
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        step()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()               # the graph's pool replaces the eager blocks
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        g_out = step()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        sec = timed(lambda: graph.replay(), 3, stream)
+    eager_out = step()
+    torch.cuda.synchronize()
+    graph_vs_eager = max(((a - b).abs().max() / b.abs().max().clamp_min(1e-30)).item()
+                         for a, b in zip(g_out, eager_out))
+    del graph, g_out, eager_out
+    torch.cuda.empty_cache()
     with torch.no_grad():
         first = [p.detach()[:1] for p in params]
         n_gates = len(tc.gates(*first))
         n_apps = len(tc.gates(*first, fuse_qudits=args.fuse_qudits))
     res = {'qubits': 12, 'layers': 40, 'params_per_set': tc.n_params, 'param_sets': sets,
            'fwd_adjoint_s': round(sec, 4), 'param_sets_per_s': round(sets / sec, 2),
+           'timing': 'HIP-graph replays', 'fwd_adjoint_s_eager': round(sec_eager, 4),
+           'graph_vs_eager_max_rel_diff': float('%.3e' % graph_vs_eager),
            'gates': n_gates, 'applications': n_apps, 'fuse_qudits': args.fuse_qudits,
            'checkpoint_every': 8, 'free_gb_at_start': round(free0 / 1e9, 1)}
     log('differentiable: %s' % res)
