@@ -247,6 +247,7 @@ def bench_differentiable(args):
     eagerly and as replays of the step captured in a HIP graph."""
     import torch
     from quantumsim_amd.workloads import TrainableCircuit
+    from quantumsim_amd.autograd import capture_step
     tc = TrainableCircuit(n=12, layers=40)
     sets = args.param_sets
     free0 = torch.cuda.mem_get_info()[0]
@@ -266,16 +267,7 @@ def bench_differentiable(args):
     sec_eager = timed(step, 2, torch.cuda.current_stream())
     # the same step captured once in a HIP graph (on a stream of its own,
     # default priority) and replayed: no host launch gaps
-    stream = torch.cuda.Stream()
-    stream.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(stream):
-        step()
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()               # the graph's pool replaces the eager blocks
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph, stream=stream):
-        g_out = step()
-    torch.cuda.synchronize()
+    graph, stream, g_out = capture_step(step)
     with torch.cuda.stream(stream):
         sec = timed(lambda: graph.replay(), 3, stream)
     eager_out = step()

@@ -30,7 +30,7 @@ from . import _native as nat
 from .autotune import get_tuner
 from .bases import general
 
-__all__ = ['apply', 'run_circuit', 'run_schedule', 'diagonal', 'Gate', 'qubit_basis_tensor', 'kraus_ptm',
+__all__ = ['apply', 'run_circuit', 'run_schedule', 'capture_step', 'diagonal', 'Gate', 'qubit_basis_tensor', 'kraus_ptm',
            'rx_ptm', 'rz_ptm', 'damping_ptm', 'cz_ptm', 'kron_ptm', 'chain_ptm']
 
 
@@ -232,6 +232,29 @@ class _Circuit(torch.autograd.Function):
                         g = g.sum(0, keepdim=True)
                     grads[j] = g.to(ptms[j].dtype)
         return (gy, None, *grads)
+
+
+def capture_step(fn, warmup=1, stream=None):
+    """Record one call of ``fn`` (a forward + backward built from this
+    module's operations and torch ops, with no host synchronisation) in a
+    HIP graph.  ``fn`` runs ``warmup`` times eagerly on ``stream`` first (a
+    new stream of default priority if None), which sizes every buffer and
+    scratch area, then the eager blocks are handed back and the call is
+    captured.  Returns (graph, stream, outputs): ``graph.replay()`` on
+    ``stream`` recomputes ``outputs`` in place."""
+    if stream is None:
+        stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        for _ in range(max(1, int(warmup))):
+            fn()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        outputs = fn()
+    torch.cuda.synchronize()
+    return graph, stream, outputs
 
 
 def run_circuit(x0, gates, shape, checkpoint_every=8):

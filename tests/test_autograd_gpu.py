@@ -525,3 +525,30 @@ def test_circuit_batch_side_stream_equals_one_stream():
             assert torch.equal(a, b)
         for a, b in zip(gr, g1):
             assert torch.equal(a, b)
+
+
+def test_capture_step_trainable_circuit():
+    """A differentiable step (trainable circuit in 3-qubit blocks, KL loss,
+    gradients of every parameter) captured with capture_step: replays equal
+    the eager step, and follow parameter updates made in place."""
+    from quantumsim_amd.autograd import capture_step
+    tc = TrainableCircuit(n=6, layers=8)
+    ang, t1, t2 = tc.initial_params(4, seed=8, device=DEV, dtype=torch.float64)
+    with torch.no_grad():
+        target = tc.populations(ang + 0.1, t1, t2, torch.float32)
+    params = [ang.requires_grad_(), t1.requires_grad_(), t2.requires_grad_()]
+
+    def step():
+        p = tc.populations(*params, torch.float32, checkpoint_every=4, fuse_qudits=3)
+        return torch.autograd.grad(tc.kl(target, p).sum(), params)
+
+    graph, stream, out = capture_step(step)
+    for rep in range(3):
+        with torch.no_grad():
+            ang.add_(0.01)                      # in place: the graph reads the same memory
+        with torch.cuda.stream(stream):
+            graph.replay()
+        torch.cuda.synchronize()
+        ref = step()
+        for a, b in zip(out, ref):
+            assert torch.equal(a, b), rep
