@@ -242,17 +242,30 @@ __global__ void __launch_bounds__(kBlock)
   int64_t gin, gout;
   tile_base(tg, blockIdx.x, gin, gout);
 
-  // stage the input runs
+  // stage the input runs: kStage vectors per thread are loaded before any
+  // is written to LDS, so that many loads are in flight per wave
+  constexpr int kStage = 4;
   for (int a = 0; a < tg.runs_in; ++a) {
     const T* src = in + gin + tg.run_in[a];
     const int base = a * tg.L_in;
-    for (int e = tid * VEC; e < tg.L_in; e += kBlock * VEC) {
-      const V val = gload<NT>(reinterpret_cast<const V*>(src + e));
-      if constexpr (VEC == 1) {
-        lin[lds_pad(base + e)] = val;
-      } else {
+    for (int e0 = tid * VEC; e0 < tg.L_in; e0 += kStage * kBlock * VEC) {
+      V val[kStage];
 #pragma unroll
-        for (int c = 0; c < VEC; ++c) lin[lds_pad(base + e + c)] = val[c];
+      for (int u = 0; u < kStage; ++u) {
+        const int e = e0 + u * kBlock * VEC;
+        if (e < tg.L_in) val[u] = gload<NT>(reinterpret_cast<const V*>(src + e));
+      }
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int e = e0 + u * kBlock * VEC;
+        if (e < tg.L_in) {
+          if constexpr (VEC == 1) {
+            lin[lds_pad(base + e)] = val[u];
+          } else {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) lin[lds_pad(base + e + c)] = val[u][c];
+          }
+        }
       }
     }
   }
