@@ -518,8 +518,8 @@ struct LdsPlan {
 
 // Plans the LDS-tiled variant; returns false when the direct kernel is the
 // better (or only) choice.
-// Orbits per LDS tile: about 8-16 KB of staged input (measured best on
-// MI355X for 14-qubit vectors), at most 4 orbits per thread.
+// Orbits per LDS tile: about 8 KB of staged input (measured best on MI355X
+// for 14-qubit vectors against 4 and 16 KB), at most 4 orbits per thread.
 // QSV_LDS_ORBITS overrides it for tuning.
 constexpr int kLdsMaxOrbits = 4 * kBlock;
 static int64_t lds_tile_orbits(int64_t dmax, int elem_bytes) {
@@ -527,8 +527,7 @@ static int64_t lds_tile_orbits(int64_t dmax, int elem_bytes) {
     const char* e = getenv("QSV_LDS_ORBITS");
     return e ? int64_t(atoll(e)) : int64_t(0);
   }();
-  // measured optimum: 16 KB tiles, except 1-qubit float64 PTMs (8 KB)
-  const int64_t tile_bytes = (elem_bytes == 8 && dmax <= 4) ? 8192 : 16384;
+  constexpr int64_t tile_bytes = 8192;
   int64_t x = forced > 0 ? forced : tile_bytes / (dmax * elem_bytes);
   if (dmax > 4 && x > 2 * kBlock) x = 2 * kBlock;  // 16-wide register tiles: <= 2 orbits
   return x < 16 ? int64_t(16) : (x > kLdsMaxOrbits ? int64_t(kLdsMaxOrbits) : x);
@@ -738,7 +737,9 @@ static bool make_lds_plan(const Plan& p, int elem_bytes, LdsPlan& lp,
   const int64_t n_in = int64_t(tg.runs_in) * tg.L_in, n_out = int64_t(tg.runs_out) * tg.L_out;
   const int64_t elems = (n_in + n_in / 16 + 1) + (square ? 0 : n_out + n_out / 16 + 1);
   lp.lds_bytes = size_t(elems * elem_bytes);
-  lp.nt = false;  // measured: the staged runs are faster with normal caching
+  // the direct kernel's rule: nontemporal loads and stores for vectors that
+  // do not stay in the caches between gates (the staged runs are whole lines)
+  lp.nt = p.nt;
   lp.batch = p.batch;
   tg.bs_in = p.n_in;
   tg.bs_out = p.n_out;
@@ -754,14 +755,24 @@ template <typename T, int VEC, int CH, int ORB, class Op>
 static int launch_lds(const LdsPlan& lp, const Op& op, const void* in, void* out,
                       hipStream_t s) {
   if (lp.tg.n_tiles == 0 || lp.batch == 0) return QSV_OK;
-  if (lp.tg.din == CH && lp.tg.dout == CH)
-    hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, true, Op>),
-                       dim3(lp.tg.n_tiles, lp.batch), dim3(kBlock), lp.lds_bytes, s,
-                       static_cast<const T*>(in), static_cast<T*>(out), lp.tg, op);
+  const dim3 grid(lp.tg.n_tiles, lp.batch);
+  const bool full = lp.tg.din == CH && lp.tg.dout == CH;
+  if (lp.nt && full)
+    hipLaunchKernelGGL((k_apply_lds<T, VEC, true, CH, ORB, true, Op>), grid, dim3(kBlock),
+                       lp.lds_bytes, s, static_cast<const T*>(in), static_cast<T*>(out), lp.tg,
+                       op);
+  else if (lp.nt)
+    hipLaunchKernelGGL((k_apply_lds<T, VEC, true, CH, ORB, false, Op>), grid, dim3(kBlock),
+                       lp.lds_bytes, s, static_cast<const T*>(in), static_cast<T*>(out), lp.tg,
+                       op);
+  else if (full)
+    hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, true, Op>), grid, dim3(kBlock),
+                       lp.lds_bytes, s, static_cast<const T*>(in), static_cast<T*>(out), lp.tg,
+                       op);
   else
-    hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, false, Op>),
-                       dim3(lp.tg.n_tiles, lp.batch), dim3(kBlock), lp.lds_bytes, s,
-                       static_cast<const T*>(in), static_cast<T*>(out), lp.tg, op);
+    hipLaunchKernelGGL((k_apply_lds<T, VEC, false, CH, ORB, false, Op>), grid, dim3(kBlock),
+                       lp.lds_bytes, s, static_cast<const T*>(in), static_cast<T*>(out), lp.tg,
+                       op);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
 }
