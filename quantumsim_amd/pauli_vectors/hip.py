@@ -61,15 +61,19 @@ def ctypes_void(v):
 
 
 class _Scratch:
-    """Per-device grow-only device buffers (PTM tables, partial sums).
+    """Grow-only device buffers (PTM tables, partial sums), one per device,
+    stream and purpose.
 
-    Buffers are used in stream order, so one buffer per purpose is enough."""
+    Calls on one stream use a buffer in stream order; calls issued on
+    different streams get different buffers, so they cannot overwrite a
+    table another stream's kernel is still reading, and a buffer that grows
+    is handed back to the allocator on the stream it was used on."""
 
     _bufs = {}
 
     @classmethod
     def get(cls, device, purpose, nbytes):
-        key = (device, purpose)
+        key = (device, torch.cuda.current_stream(device).cuda_stream, purpose)
         t = cls._bufs.get(key)
         if t is None or t.numel() < nbytes:
             t = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)

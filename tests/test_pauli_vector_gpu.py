@@ -451,3 +451,35 @@ def test_14q_orthogonal_round_trip(dtype, tol, rng):
     # the padded layout: strides of the outer axes are not powers of two
     if dtype == 'float64':
         assert v._pstr[0] != 4 ** (n - 1)
+
+
+def test_two_streams_dense_ptms_match_one_stream(rng):
+    """Vectors driven on two torch streams at once, with dense PTMs that are
+    staged through a device workspace (64 x 64 on MFMA, 81 x 81 qutrit
+    pairs in the table kernel): every result equals the same calls made
+    one after another on one stream (each stream has its own workspace)."""
+    import torch
+    cases = [([bases.general(2)] * 6, [4] * 6, (1, 2, 4)),
+             ([bases.general(3)] * 4, [9] * 4, (0, 2))]
+    for bs, shape, targets in cases:
+        d = math.prod(shape[t] for t in targets)
+        pvs = [rng.standard_normal(shape) for _ in range(2)]
+        ptms = [[rng.standard_normal([shape[t] for t in targets] * 2) / d for _ in range(4)]
+                for _ in range(2)]
+        ref = []
+        for pv, seq in zip(pvs, ptms):
+            v = qa.PauliVectorHIP(bs, pv)
+            for p in seq:
+                v.apply_ptm(p, *targets)
+            ref.append(v.to_pv())
+        vs = [qa.PauliVectorHIP(bs, pv) for pv in pvs]
+        streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        for s in streams:
+            s.wait_stream(torch.cuda.current_stream())
+        for r in range(4):
+            for v, s, seq in zip(vs, streams, ptms):
+                with torch.cuda.stream(s):
+                    v.apply_ptm(seq[r], *targets)
+        torch.cuda.synchronize()
+        for v, want in zip(vs, ref):
+            assert np.array_equal(v.to_pv(), want), (shape, targets)
