@@ -552,3 +552,26 @@ def test_capture_step_trainable_circuit():
         ref = step()
         for a, b in zip(out, ref):
             assert torch.equal(a, b), rep
+
+
+def test_batched_apply_large_batch_equals_per_state():
+    """A batch whose states exceed the nontemporal threshold together (16
+    float32 states of 11 qubits: 0.5 GB read and written) but not one by one:
+    the batched launch (nontemporal accesses) matches per-state launches
+    (cached accesses) for per-state 16 x 16 PTMs on middle and inner qubits
+    (direct and LDS-tiled kernels) and dense 64 x 64 blocks (MFMA)."""
+    from quantumsim_amd.autograd import _apply_raw
+    n, B = 11, 16
+    shape = [4] * n
+    g = torch.Generator().manual_seed(5)
+    x = torch.randn(B, 4 ** n, generator=g).to(DEV, torch.float32)
+    for targets in ([4, 5], [9, 10], [3, 4, 5], [8, 9, 10]):
+        D = 4 ** len(targets)
+        ptm = (torch.randn(B, D, D, generator=g, dtype=torch.float64) / D).to(DEV)
+        dims_out = [4] * len(targets)
+        y = _apply_raw(x, ptm, shape, targets, dims_out)
+        for b in range(B):
+            yb = _apply_raw(x[b:b + 1].contiguous(), ptm[b:b + 1], shape, targets, dims_out)
+            scale = float(yb.abs().max())
+            err = float((y[b] - yb[0]).abs().max())
+            assert err <= 1e-6 * scale, (targets, b, err, scale)
