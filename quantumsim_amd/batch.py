@@ -51,6 +51,8 @@ from .workloads import (rotation_ptms, damping_ptms, fused_layer_gates, layer_ta
 __all__ = ['CircuitBatch', 'plain_forward_backward', 'choose_checkpoint_interval']
 
 CHECKPOINT_INTERVALS = (1, 2, 4, 8, 16)
+# share of the free device memory the kept states may take by default
+DEFAULT_BUDGET_FRACTION = 0.4
 
 
 def choose_checkpoint_interval(kept_bytes, budget, intervals=CHECKPOINT_INTERVALS):
@@ -137,10 +139,13 @@ class CircuitBatch:
         self.ptable = self._persistent_table() if self.small else None
         # checkpoint interval of the batched launches: None = the shortest
         # of CHECKPOINT_INTERVALS whose kept states fit checkpoint_budget
-        # bytes (default: 60 % of the device memory free now)
+        # bytes (default: DEFAULT_BUDGET_FRACTION of the device memory free
+        # now, leaving room for other users of the device; a caller that
+        # owns the device passes a larger budget)
         if checkpoint_every is None:
             if checkpoint_budget is None:
-                checkpoint_budget = 0.6 * torch.cuda.mem_get_info(self.device)[0]
+                checkpoint_budget = DEFAULT_BUDGET_FRACTION * torch.cuda.mem_get_info(
+                    self.device)[0]
             checkpoint_every = choose_checkpoint_interval(self.kept_bytes, checkpoint_budget)
         self.k = max(1, int(checkpoint_every))
 

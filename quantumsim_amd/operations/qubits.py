@@ -52,16 +52,45 @@ def cnot():
     return from_kraus(np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]]), _B2)
 
 
-def amp_damping(total_rate):
-    g = total_rate
-    return from_kraus(np.array([[[1, 0], [0, np.sqrt(1 - g)]],
-                                [[0, np.sqrt(g)], [0, 0]]]), _B1)
+def amp_damping(total_rate=None, *, exc_rate=None, damp_rate=None):
+    """Amplitude damping (qubits.py:182-198).  ``total_rate``: decay to |0>
+    with probability ``total_rate`` (Kraus form, ``general(2)`` basis).
+    Otherwise generalised amplitude damping with excitation rate
+    ``exc_rate`` and decay rate ``damp_rate``, given as its PTM in the
+    normalised Pauli basis ``gell_mann(2)`` = (I, X, Y, Z) / sqrt(2): X and
+    Y shrink by sqrt(1 - r), Z by 1 - r (r = exc_rate + damp_rate) and
+    moves by damp_rate - exc_rate towards |0>."""
+    if total_rate is not None:
+        g = total_rate
+        return from_kraus(np.array([[[1, 0], [0, np.sqrt(1 - g)]],
+                                    [[0, np.sqrt(g)], [0, 0]]]), _B1)
+    if exc_rate is None or damp_rate is None:
+        raise ValueError('Either the total_rate or both the exc_rate and damp_rate '
+                         'must be provided')
+    r = exc_rate + damp_rate
+    ptm = np.zeros((4, 4))
+    ptm[0, 0] = 1.0
+    ptm[1, 1] = ptm[2, 2] = np.sqrt(1 - r)
+    ptm[3, 3] = 1 - r
+    ptm[3, 0] = damp_rate - exc_rate
+    return PTMOp(ptm, (bases.gell_mann(2),))
 
 
-def phase_damping(total_rate):
-    lam = total_rate
-    return from_kraus(np.array([[[1, 0], [0, np.sqrt(1 - lam)]],
-                                [[0, 0], [0, np.sqrt(lam)]]]), _B1)
+def phase_damping(total_rate=None, *, x_deph_rate=None, y_deph_rate=None, z_deph_rate=None):
+    """Phase damping (qubits.py:201-214).  ``total_rate``: the Kraus form
+    that shrinks the coherences by sqrt(1 - total_rate) (``general(2)``
+    basis).  Otherwise Pauli-axis dephasing: X, Y, Z of the normalised
+    Pauli basis ``gell_mann(2)`` shrink by 1 - x_deph_rate, 1 - y_deph_rate,
+    1 - z_deph_rate."""
+    if total_rate is not None:
+        lam = total_rate
+        return from_kraus(np.array([[[1, 0], [0, np.sqrt(1 - lam)]],
+                                    [[0, 0], [0, np.sqrt(lam)]]]), _B1)
+    rates = (x_deph_rate, y_deph_rate, z_deph_rate)
+    if any(r is None for r in rates):
+        raise ValueError('Either the total_rate or the dephasing rates along each of '
+                         'the three axis must be provided')
+    return PTMOp(np.diag([1.0] + [1.0 - r for r in rates]), (bases.gell_mann(2),))
 
 
 def amp_phase_damping(damp_rate, deph_rate):

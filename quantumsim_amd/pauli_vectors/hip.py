@@ -226,6 +226,18 @@ class PauliVectorHIP(PauliVectorBase):
     def _elem_bytes(self):
         return 8 if self._dtype == torch.float64 else 4
 
+    def _used_here(self, *tensors):
+        """Mark the vector's own buffers as used by work just queued on the
+        current stream.  A buffer allocated on another stream would otherwise
+        go back to that stream's pool when it is dropped (a spare replaced by
+        a larger one, the vector deleted) and could be handed out again while
+        this stream's kernels still read or write it; the caching allocator
+        ignores the mark for a buffer's own allocation stream."""
+        cur = torch.cuda.current_stream(self._device)
+        for t in tensors:
+            if t is not None:
+                t.record_stream(cur)
+
     def _take_spare(self, n):
         sp = self._spare
         if sp is None or sp.numel() < n:
@@ -308,6 +320,7 @@ class PauliVectorHIP(PauliVectorBase):
         if rc != 0:
             self._spare = out
             nat.check(rc, 'apply_ptm')
+        self._used_here(self._store, out)
         self._swap_in(out, extent, new_shape, new_str)
 
     # ------------------------------------------------------------ readouts
@@ -342,6 +355,7 @@ class PauliVectorHIP(PauliVectorBase):
                                      nat.i64(self._pstr), nat.i64(plan.take_dims),
                                      plan.idx.data_ptr(), nat.i32(plan.offsets), s)
         nat.check(rc, 'diagonal')
+        self._used_here(self._store)
         cur = cur[:n_take]
         dims = list(plan.take_dims)
         for q, a in enumerate(plan.axes):
@@ -401,6 +415,7 @@ class PauliVectorHIP(PauliVectorBase):
             len(self._shape), nat.i64(self._shape), nat.i64(self._pstr), nat.i64(out_dims),
             table.data_ptr(), nat.i32(offsets), _stream(self._device))
         nat.check(rc, 'multitake')
+        self._used_here(self._store)
         res = out[:n_out].view(tuple(out_dims))
         return res.cpu().numpy() if get_data else res
 
@@ -456,6 +471,7 @@ class PauliVectorHIP(PauliVectorBase):
                 nat.i32([1]), np.ascontiguousarray(t).ctypes.data_as(nat._c_f64p), ws,
                 ws_bytes, s)
             nat.check(rc, 'partial_trace')
+            self._used_here(cur)
             cur, shape, strides = nxt[:math.prod(new_shape)], new_shape, None
         kept_shape = tuple(self._shape[i] for i in range(self.n_qubits) if i in qubits)
         data = cur.reshape(kept_shape) if traced else self.to_torch()
@@ -470,6 +486,7 @@ class PauliVectorHIP(PauliVectorBase):
             rc = nat.lib().qsv_scale(self._code(), self._data.data_ptr(), self._data.numel(),
                                      float(tr ** -1), _stream(self._device))
             nat.check(rc, 'renormalize')
+            self._used_here(self._store)
         else:
             warnings.warn(
                 "Density matrix trace is 0; likely your further computation "
@@ -509,5 +526,7 @@ class PauliVectorHIP(PauliVectorBase):
     def copy(self):
         """Deep copy on the same device (the bases list is new, the basis
         objects are shared, as in numpy.py:119-120)."""
-        return self.__class__(self.bases, self.to_torch(), force=True,
-                              dtype=self._dtype, device=self._device)
+        new = self.__class__(self.bases, self.to_torch(), force=True,
+                             dtype=self._dtype, device=self._device)
+        self._used_here(self._store)
+        return new

@@ -64,6 +64,14 @@ class CircuitTable:
             traj += len(gates) * 4 ** n
             pop += 2 ** n
             self.max_n = max(self.max_n, n)
+        # every gate owns its PTM slot: the adjoint writes (not adds) dL/dP
+        # there, so slots that overlap would lose gradients
+        used = sorted((int(r['ptm']), int(r['ptm']) + 16 ** int(r['k']))
+                      for r in self.gates[:gi])
+        for (a0, a1), (b0, _) in zip(used[:-1], used[1:]):
+            if b0 < a1:
+                raise ValueError('PTM slots [%d, %d) and [%d, ...) overlap: every gate '
+                                 'needs a slot of its own' % (a0, a1, b0))
         self.n_gates = gi
         self.traj_elems = traj
         self.pop_elems = pop

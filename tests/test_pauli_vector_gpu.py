@@ -483,3 +483,40 @@ def test_two_streams_dense_ptms_match_one_stream(rng):
         torch.cuda.synchronize()
         for v, want in zip(vs, ref):
             assert np.array_equal(v.to_pv(), want), (shape, targets)
+
+
+def test_side_stream_growing_subbasis_vector_matches_oracle(rng):
+    """A vector made on the default stream and driven on a side stream with
+    PTMs that grow its extent (a subbasis qubit mapped back to the full
+    basis), so buffers allocated on the default stream are replaced while
+    the side stream's kernels may still use them; the default stream then
+    allocates and fills memory of the same sizes before anything is
+    synchronised.  The result equals the oracle's."""
+    import torch
+    sub = bases.general(2).subbasis([0, 1])
+    full = bases.general(2)
+    bs = [sub, sub] + [full] * 5
+    shape = [b.dim_pauli for b in bs]
+    pv = rng.standard_normal(shape)
+    grow = rng.standard_normal((4, 2))
+    sq = [rng.standard_normal((4, 4, 4, 4)) / 4 for _ in range(3)]
+    o = PauliVectorNumpy(_oracle_bases(bs), pv)
+    for q in (0, 1):
+        o.apply_ptm(grow, q)
+    for p in sq:
+        o.apply_ptm(p, 2, 0)
+    want = o.to_pv()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    v = qa.PauliVectorHIP(bs, pv)
+    with torch.cuda.stream(side):
+        for q in (0, 1):
+            v.apply_ptm(grow, q)          # extent grows: the old buffers are dropped
+        for p in sq:
+            v.apply_ptm(p, 2, 0)
+    junk = [torch.full((4 ** 7,), float('nan'), dtype=torch.float64, device=v.device)
+            for _ in range(6)]
+    side.synchronize()
+    torch.cuda.synchronize()
+    del junk
+    assert rel_err(v.to_pv(), want) < TOL64

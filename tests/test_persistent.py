@@ -75,9 +75,11 @@ def test_table_rejects_bad_input():
         CircuitTable([(3, [((1, 1), 0)])], 256)         # repeated target
     with pytest.raises(ValueError):
         CircuitTable([(3, [((0, 1), 10)])], 256)        # PTM beyond the pool
-    t = CircuitTable([(2, [((0,), 0), ((1,), 8)])], 32)  # two gates sharing PTM entries
-    with pytest.raises(ValueError):
-        emulate(t, np.zeros(32))
+    with pytest.raises(ValueError):                      # two gates sharing PTM entries
+        CircuitTable([(2, [((0,), 0), ((1,), 8)])], 32)
+    with pytest.raises(ValueError):                      # ... across circuits too
+        CircuitTable([(2, [((0,), 0)]), (2, [((0, 1), 0)])], 256)
+    CircuitTable([(2, [((0,), 0), ((1,), 16)])], 32)     # adjacent slots are fine
 
 
 def test_wide_circuits_and_swizzle():

@@ -74,3 +74,44 @@ def test_rotation_ptms_gradient():
     g1 = torch.autograd.grad((rotation_ptms(a) * w).sum(), a)[0]
     g2 = torch.autograd.grad(((rz_ptm(a[:, 1]) @ rx_ptm(a[:, 0])) * w).sum(), a)[0]
     assert (g1 - g2).abs().max().item() < 1e-13
+
+
+def _in_general(op):
+    from quantumsim_amd import bases
+    return op.set_bases((bases.general(2),), (bases.general(2),)).ptm
+
+
+def test_damping_keyword_forms():
+    """The keyword forms of amp_damping / phase_damping
+    (quantumsim/operations/qubits.py:182-214) are PTMs in gell_mann(2):
+    converted to general(2) (through the oracle's ptm_convert_basis as well)
+    they equal the Kraus forms where the two meet, and the generalised
+    amplitude damping's fixed point has the populations
+    (damp, exc) / (damp + exc)."""
+    from quantumsim_amd.operations import qubits as qg
+    from quantumsim_amd import bases
+    g, lam = 0.13, 0.27
+    ad = qg.amp_damping(exc_rate=0.0, damp_rate=g)
+    assert ad.bases_in[0] == bases.gell_mann(2)
+    np.testing.assert_allclose(_in_general(ad), qg.amp_damping(g).ptm, atol=1e-15)
+    GM, GE = [ob.gell_mann(2)], [ob.general(2)]
+    np.testing.assert_allclose(oa.ptm_convert_basis(ad.ptm, GM, GM, GE, GE),
+                               qg.amp_damping(g).ptm, atol=1e-15)
+    c = 1 - np.sqrt(1 - lam)
+    pd = qg.phase_damping(x_deph_rate=c, y_deph_rate=c, z_deph_rate=0.0)
+    np.testing.assert_allclose(_in_general(pd), qg.phase_damping(lam).ptm, atol=1e-15)
+    # excitation and decay together: the populations relax to (d, e) / (d + e)
+    e, d = 0.05, 0.2
+    P = _in_general(qg.amp_damping(exc_rate=e, damp_rate=d))
+    x = np.array([0.0, 1.0, 0.0, 0.0])             # |1><1| in general(2)
+    for _ in range(400):
+        x = P @ x
+    np.testing.assert_allclose(x[:2], [d / (d + e), e / (d + e)], atol=1e-12)
+    np.testing.assert_allclose(x[2:], 0, atol=1e-12)
+    # the Pauli-axis rates act on X, Y, Z separately
+    P = qg.phase_damping(x_deph_rate=0.1, y_deph_rate=0.2, z_deph_rate=0.3).ptm
+    np.testing.assert_allclose(np.diag(P), [1, 0.9, 0.8, 0.7])
+    with pytest.raises(ValueError):
+        qg.amp_damping(exc_rate=0.1)
+    with pytest.raises(ValueError):
+        qg.phase_damping(x_deph_rate=0.1, y_deph_rate=0.1)
