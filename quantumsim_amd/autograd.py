@@ -27,6 +27,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import ledger as _ledger
 from .autotune import get_tuner
 from .bases import general
 
@@ -70,11 +71,17 @@ def _apply_raw(x, ptm, shape, targets, dims_out, out=None):
     ptm = ptm.to(dtype=x.dtype).contiguous()
     pstride = 0 if ptm.shape[0] == 1 else ptm.shape[1] * ptm.shape[2]
     tile = get_tuner().apply_tile(x.dtype, B, shape, targets, dims_out)
+    led = _ledger.active()
+    ev = led.begin() if led else None
     rc = nat.lib().qsv_apply_ptm_batched_tiled(
         nat.dtype_code(x.dtype), x.data_ptr(), y.data_ptr(), B, len(shape), nat.i64(shape),
         len(targets), nat.i32(targets), nat.i32(dims_out), ptm.data_ptr(), pstride,
         None if tile is None else nat.ctypes.byref(tile), _stream())
     nat.check(rc, 'apply')
+    if led:
+        dense = ptm.shape[1] == 64 and ptm.shape[2] == 64
+        led.end(ev, _ledger.APPLY_DENSE64 if dense else _ledger.APPLY_SMALL,
+                x.element_size() * (B * (x.shape[1] + n_out) + ptm.numel()))
     return y
 
 
@@ -97,11 +104,17 @@ def _ptm_grad_raw(x, ybar, shape, targets, dims_out, out=None):
         assert (out.shape[0] >= B and tuple(out.shape[1:]) == (dout, din)
                 and out.dtype == x.dtype and out.is_contiguous())
     tile = get_tuner().grad_tile(x.dtype, B, shape, targets, dims_out)
+    led = _ledger.active()
+    ev = led.begin() if led else None
     rc = L.qsv_ptm_grad_tiled(nat.dtype_code(x.dtype), x.data_ptr(), ybar.data_ptr(), B,
                               len(shape), nat.i64(shape), k, nat.i32(targets), nat.i32(dims_out),
                               out.data_ptr(), 0, buf.data_ptr(), ws,
                               None if tile is None else nat.ctypes.byref(tile), _stream())
     nat.check(rc, 'ptm_grad')
+    if led:
+        wide = din > 16 or dout > 16
+        led.end(ev, _ledger.GRAD_DENSE64 if wide else _ledger.GRAD_SMALL,
+                x.element_size() * B * (x.shape[1] + ybar.shape[1] + dout * din))
     return out
 
 

@@ -445,16 +445,22 @@ class CircuitBatch:
         return [torch.tensor(g.standard_normal((len(gr.members), 2 ** gr.n)),
                              dtype=torch.float64, device=self.device) for gr in self.groups]
 
-    def capture(self, weights, autotune=True):
+    def capture(self, weights, autotune=True, ledger=None):
         """Record one forward + adjoint step in a HIP graph (after an eager
         warm-up step that sizes every buffer and, with ``autotune``, times
         the launch shapes of every batched apply and PTM gradient of the
-        step: see quantumsim_amd.autotune); replay() re-runs it."""
+        step: see quantumsim_amd.autotune); replay() re-runs it.  A
+        ``ledger.Ledger`` given here counts (and times) the warm-up step's
+        launches: the step the graph records, with the tuned shapes."""
         if autotune:
             with tuning():
                 self.step(weights)               # tunes each new geometry once
             torch.cuda.synchronize(self.device)
-        self.step(weights)                       # warm-up: allocations, tables
+        if ledger is not None:
+            with ledger:
+                self.step(weights)               # warm-up, counted
+        else:
+            self.step(weights)                   # warm-up: allocations, tables
         torch.cuda.synchronize(self.device)
         # the graph allocates from a pool of its own: hand the blocks the
         # eager steps cached back first (as large as the graph's own needs

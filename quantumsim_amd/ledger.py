@@ -1,0 +1,100 @@
+"""Algorithmic HBM bytes of the library's batched launches, by kernel family.
+
+While a ``Ledger`` is active (``with Ledger() as led: ...``), every launch
+the differentiable and batched paths make through libqsv adds the bytes its
+algorithm has to move:
+
+* apply (``qsv_apply_ptm_batched``): (N_in + N_out) * s per state plus the
+  PTMs read;
+* PTM gradient (``qsv_ptm_grad``): both operand states, 2 * N * s per
+  state for a square PTM, plus the gradient written;
+* persistent circuit kernels: the trajectory (the state before every gate,
+  written forward and read back by the adjoint), the PTM pool, the float64
+  populations and their gradients; the states themselves live in LDS.
+
+With ``timed=True`` each launch is also bracketed by HIP events on the
+stream it is queued on, so one instrumented eager step gives every family's
+GPU time and rate.  Off (the default) it costs one global lookup per launch.
+No counterpart in the reference.
+"""
+import torch
+
+__all__ = ['Ledger', 'active', 'APPLY_DENSE64', 'APPLY_SMALL', 'GRAD_DENSE64', 'GRAD_SMALL',
+           'PERSISTENT_FWD', 'PERSISTENT_ADJ', 'KERNELS']
+
+APPLY_DENSE64 = 'apply_dense64'
+APPLY_SMALL = 'apply_small'
+GRAD_DENSE64 = 'grad_dense64'
+GRAD_SMALL = 'grad_small'
+PERSISTENT_FWD = 'persistent_forward'
+PERSISTENT_ADJ = 'persistent_adjoint'
+
+# the HIP kernels behind each family (rocprofv3 names)
+KERNELS = {
+    APPLY_DENSE64: 'k_fused64 (64 x 64 PTMs on MFMA)',
+    APPLY_SMALL: 'k_apply_small / k_apply_lds (PTMs up to 16 x 16)',
+    GRAD_DENSE64: 'k_ptm_grad64 + k_ptm_grad_final',
+    GRAD_SMALL: 'k_ptm_grad + k_ptm_grad_final',
+    PERSISTENT_FWD: 'k_circuits_fwd',
+    PERSISTENT_ADJ: 'k_circuits_adj',
+}
+
+_ACTIVE = None
+
+
+def active():
+    """The ledger in force, or None."""
+    return _ACTIVE
+
+
+class Ledger:
+    def __init__(self, timed=True):
+        self.timed = timed
+        self.rows = {}        # family -> [launches, bytes, [(start, end) events]]
+
+    def __enter__(self):
+        global _ACTIVE
+        self._prev = _ACTIVE
+        _ACTIVE = self
+        return self
+
+    def __exit__(self, *exc):
+        global _ACTIVE
+        _ACTIVE = self._prev
+        return False
+
+    def begin(self):
+        """Call right before a launch; returns its start event (or None)."""
+        if not self.timed:
+            return None
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return ev
+
+    def end(self, start, family, nbytes):
+        """Call right after the launch that ``start`` belongs to."""
+        row = self.rows.setdefault(family, [0, 0, []])
+        row[0] += 1
+        row[1] += int(nbytes)
+        if start is not None:
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record()
+            row[2].append((start, ev))
+
+    def total_bytes(self):
+        return sum(r[1] for r in self.rows.values())
+
+    def summary(self, peak_gbs=None):
+        """{family: {launches, bytes, ms, GBs[, frac]}} (waits for the GPU)."""
+        torch.cuda.synchronize()
+        out = {}
+        for fam, (n, b, evs) in sorted(self.rows.items()):
+            row = {'kernel': KERNELS.get(fam, fam), 'launches': n, 'bytes': b}
+            if evs:
+                ms = sum(a.elapsed_time(c) for a, c in evs)
+                row['ms'] = round(ms, 3)
+                row['GBs'] = round(b / ms / 1e6, 1) if ms > 0 else None
+                if peak_gbs and row['GBs']:
+                    row['frac'] = round(row['GBs'] / peak_gbs, 4)
+            out[fam] = row
+        return out

@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import ledger as _ledger
 
 __all__ = ['CIRCUIT_DTYPE', 'GATE_DTYPE', 'CircuitTable', 'max_qubits', 'run', 'emulate']
 
@@ -99,10 +100,16 @@ class _Persistent(torch.autograd.Function):
         pool_t = pool.detach().to(dtype).contiguous()
         traj = torch.empty(max(table.traj_elems, 1), dtype=dtype, device=pool.device)
         pops = torch.empty(max(table.pop_elems, 1), dtype=torch.float64, device=pool.device)
+        led = _ledger.active()
+        ev = led.begin() if led else None
         rc = L.qsv_circuits_forward(code, len(table.circ), table.circ_dev.data_ptr(),
                                     table.gates_dev.data_ptr(), pool_t.data_ptr(),
                                     traj.data_ptr(), pops.data_ptr(), table.max_n, _stream())
         nat.check(rc, 'circuits_forward')
+        if led:
+            es = traj.element_size()
+            led.end(ev, _ledger.PERSISTENT_FWD,
+                    es * (table.ptm_elems + table.traj_elems) + 8 * table.pop_elems)
         ctx.table, ctx.dtype = table, dtype
         ctx.save_for_backward(pool_t, traj)
         return pops[:table.pop_elems]
@@ -113,11 +120,18 @@ class _Persistent(torch.autograd.Function):
         table = ctx.table
         dpops = dpops.to(torch.float64).contiguous()
         dpool = torch.zeros(pool_t.shape, dtype=torch.float64, device=pool_t.device)
+        led = _ledger.active()
+        ev = led.begin() if led else None
         rc = nat.lib().qsv_circuits_adjoint(nat.dtype_code(ctx.dtype), len(table.circ),
                                             table.circ_dev.data_ptr(), table.gates_dev.data_ptr(),
                                             pool_t.data_ptr(), traj.data_ptr(), dpops.data_ptr(),
                                             dpool.data_ptr(), table.max_n, _stream())
         nat.check(rc, 'circuits_adjoint')
+        if led:
+            es = traj.element_size()
+            led.end(ev, _ledger.PERSISTENT_ADJ,
+                    es * (table.ptm_elems + table.traj_elems)
+                    + 8 * (table.pop_elems + table.ptm_elems))
         return dpool, None, None
 
 

@@ -77,7 +77,10 @@ def main(argv=None):
     if a.what == 'batched':
         from quantumsim_amd.batch import CircuitBatch
         from quantumsim_amd.workloads import random_circuit_batch
-        cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32)
+        # the benchmark's memory budget: kept states may take 60 % of the
+        # free device memory (every state is kept)
+        cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32,
+                          checkpoint_budget=0.6 * torch.cuda.mem_get_info()[0])
         w = cb.random_weights(7)
         cb.capture(w, autotune=True)       # tuning + warm-up + capture
         torch.cuda.synchronize()

@@ -86,3 +86,147 @@ def test_hadamard_needs_the_full_basis_only_where_it_must():
 def test_rejects_non_full_ptms():
     with pytest.raises(ValueError):
         optimal_subbases([FULL] * 2, [(np.eye(2), (0,))])
+
+
+# ------------------------------------------------------------------ the
+# reference compiler's known answers (quantumsim/test/test_compiler.py),
+# restated on gate lists: which subbases the optimised circuit ends up in,
+# and that optimised and plain circuits give the same populations.
+def _kraus(kraus, full):
+    from quantumsim_amd.algebra import kraus_to_ptm
+    k = int(round(np.log(np.asarray(kraus).shape[-1]) / np.log(full.dim_hilbert)))
+    return kraus_to_ptm(np.asarray(kraus, dtype=complex), (full,) * k, (full,) * k)
+
+
+def _rx(angle, d):
+    """RX on levels 0, 1 of a d-level system, level 2 untouched
+    (quantumsim/operations/qubits.py:38-53, qutrits.py:41-56)."""
+    s, c = np.sin(angle / 2), np.cos(angle / 2)
+    u = np.eye(d, dtype=complex)
+    u[:2, :2] = [[c, -1j * s], [-1j * s, c]]
+    return u
+
+
+def _cphase_qutrit(angle, leakage):
+    """The 'legacy' qutrit CPhase (quantumsim/operations/qutrits.py:201-222):
+    exp(-i angle (1 - arcsin(sqrt(leakage)) / pi) G), G coupling |11> and
+    |02> (indices 4 and 2 of the 9 two-qutrit levels); the second qutrit
+    leaks."""
+    from scipy.linalg import expm
+    g = np.zeros((9, 9))
+    g[2, 4] = g[4, 2] = 1
+    return expm(-1j * angle * (1 - np.arcsin(np.sqrt(leakage)) / np.pi) * g)
+
+
+def _random_hermitian(dim, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((dim, dim)) + 1j * rng.standard_normal((dim, dim))
+    return (a + a.conj().T) / 2
+
+
+def _oracle_full_run(full_bases, init_supports, gates, dm):
+    """The plain circuit on the oracle: the state given by ``dm`` restricted
+    to the initial supports (the reference's from_dm into subbases), padded
+    to the full bases, every PTM applied in the full bases."""
+    from oracle import algebra as oa
+    obs = [_ob(b) for b in full_bases]
+    pv = oa.dm_to_pv(dm, obs)
+    mask = np.zeros(pv.shape, dtype=bool)
+    mask[np.ix_(*init_supports)] = True
+    o = PauliVectorNumpy(obs, np.where(mask, pv, 0.0), force=True)
+    for p, q in gates:
+        o.apply_ptm(np.asarray(p), *q)
+    return o.diagonal()
+
+
+def _oracle_opt_run(full_bases, init_supports, init, opt, dm):
+    from oracle import algebra as oa
+    pv = oa.dm_to_pv(dm, [_ob(b) for b in full_bases])[np.ix_(*init_supports)]
+    o = PauliVectorNumpy([_ob(b) for b in init], np.ascontiguousarray(pv), force=True)
+    for p, q, outs in opt:
+        o.apply_ptm(p, *q)
+        for qq, b in zip(q, outs):
+            o.bases[qq] = _ob(b)
+    return o.diagonal()
+
+
+def test_compile_cnot_shapes():
+    """test_compiler.py:108-120: CNOT in the full bases stays 4x4x4x4, on
+    two classical qubits (subbasis [0, 1]) it is 2x2x2x2, with the control
+    in |0><0| (subbasis [0]) and a full target it is 1x4x1x4."""
+    cnot = gates.cnot().ptm
+    for initial, shape in (([None, None], (4, 4, 4, 4)), ([[0, 1], [0, 1]], (2, 2, 2, 2)),
+                           ([[0], None], (1, 4, 1, 4))):
+        _, opt, _ = optimal_subbases([FULL] * 2, [(cnot, (0, 1))], initial=initial)
+        assert opt[0][0].shape == shape, (initial, opt[0][0].shape)
+
+
+@pytest.mark.parametrize('d', [2, 3])
+def test_compile_single_qubit_chain(d):
+    """test_compiler.py:122-164: two RX(pi/5) fused into one PTM equal RX(2
+    pi/5) and act identically on a Pauli vector; RX(pi) twice keeps the
+    classical subbasis [0, 1] whether it is given at the input or asked for
+    at the output."""
+    from quantumsim_amd.fusion import fuse
+    full = bases.general(d)
+    obs = [_ob(full)]
+    angle = np.pi / 5
+    rx1 = _kraus(_rx(angle, d), full)
+    rx2 = _kraus(_rx(2 * angle, d), full)
+    fused = fuse([(rx1, (0,)), (rx1, (0,))], [d * d], 2)
+    assert len(fused) == 1
+    np.testing.assert_allclose(fused[0].ptm, rx2, atol=1e-14)
+    from oracle import algebra as oa
+    pv = oa.dm_to_pv(_random_hermitian(d, 487), obs)
+    o0 = PauliVectorNumpy(obs, pv.copy())
+    o0.apply_ptm(rx1, 0)
+    o0.apply_ptm(rx1, 0)
+    o1 = PauliVectorNumpy(obs, pv.copy())
+    o1.apply_ptm(fused[0].ptm, 0)
+    np.testing.assert_allclose(o1.to_pv(), o0.to_pv(), atol=1e-14)
+    sub = full.subbasis([0, 1])
+    rxpi = _kraus(_rx(np.pi, d), full)
+    chain = [(rxpi, (0,)), (rxpi, (0,))]
+    init, opt, fin = optimal_subbases([full], chain, initial=[[0, 1]])
+    assert init[0] == sub and fin[0] == sub
+    assert all(outs[0] == sub for _, _, outs in opt)
+    init, opt, fin = optimal_subbases([full], chain, final=[[0, 1]])
+    assert init[0] == sub and fin[0] == sub
+
+
+def zz_parity_circuit():
+    """The ZZ-parity check of test_compiler.py:268-314 on three qutrits:
+    RX(-pi/2) on the ancilla 2, leaky CPhases (0, 2) and (2, 1), RX(pi/2) on
+    the ancilla, RX(pi) on both data qutrits; the data qutrits start
+    classical (subbasis [0, 1]), the ancilla in |0><0|; the readout keeps
+    the data qutrits whole and the ancilla's populations (subbasis
+    [0, 1, 2])."""
+    full = bases.general(3)
+    gl = [(_kraus(_rx(-np.pi / 2, 3), full), (2,)),
+          (_kraus(_cphase_qutrit(np.pi, 0.1), full), (0, 2)),
+          (_kraus(_cphase_qutrit(np.pi, 0.25), full), (2, 1)),
+          (_kraus(_rx(np.pi / 2, 3), full), (2,)),
+          (_kraus(_rx(np.pi, 3), full), (0,)),
+          (_kraus(_rx(np.pi, 3), full), (1,))]
+    initial = [[0, 1], [0, 1], [0]]
+    final = [None, None, [0, 1, 2]]
+    return full, gl, initial, final
+
+
+def test_compile_zz_parity_leakage_subbases():
+    """The leaking qutrit 1 ends in subbasis [0, 1, 2, 6] (|0><0|, |1><1|,
+    |2><2| and Y20), qutrit 0, which does not leak, in [0, 1], the measured
+    ancilla in [0, 1, 2] (test_compiler.py:293-303); the optimised circuit's
+    populations equal the plain circuit's (test_compiler.py:305-314)."""
+    full, gl, initial, final = zz_parity_circuit()
+    init, opt, fin = optimal_subbases([full] * 3, gl, initial=initial, final=final)
+    assert [b for b in init] == [full.subbasis([0, 1]), full.subbasis([0, 1]),
+                                 full.subbasis([0])]
+    assert fin[0] == full.subbasis([0, 1])
+    assert fin[1] == full.subbasis([0, 1, 2, 6])
+    assert list(fin[1].labels) == [full.labels[i] for i in (0, 1, 2, 6)]
+    assert fin[2] == full.subbasis([0, 1, 2])
+    dm = _random_hermitian(27, 85)
+    d_ref = _oracle_full_run([full] * 3, initial, gl, dm)
+    d_opt = _oracle_opt_run([full] * 3, initial, init, opt, dm)
+    assert np.allclose(d_opt, d_ref, atol=1e-12)
