@@ -209,6 +209,29 @@ int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batch, int rank
                  void* dptm, int accumulate, void* workspace, int64_t workspace_bytes,
                  void* stream);
 
+/* Size of the device scratch array of qsv_ptm_grad_noalloc (one per GPU,
+ * held by the library until the process ends). */
+#define QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES (16LL << 20)
+
+/* qsv_ptm_grad without a workspace argument and without any allocation:
+ * the partial tiles go to a file-scope device array of
+ * QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES bytes (fewer blocks per state than
+ * qsv_ptm_grad chooses when its tiles would not fit, which changes only
+ * the order of the float64 partial sums).  Calls no hipMalloc/hipFree.
+ * Requests whose smallest tiles (batch x dout x din x 8 bytes) exceed the
+ * array return QSV_EINVAL.  The stream must belong to the current device.
+ * Every call on one GPU shares that GPU's array: calls that overlap in
+ * time on one GPU (different streams) race; calls ordered on one stream
+ * are safe.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). */
+int qsv_ptm_grad_noalloc(int dtype, const void* x, const void* ybar, int batch, int rank,
+                         const int64_t* dims, int k, const int32_t* targets,
+                         const int32_t* dims_out_t, void* dptm, int accumulate, void* stream);
+
+/* QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES, as the library was built with it. */
+int64_t qsv_ptm_grad_noalloc_scratch_bytes(void);
+
 /* Backward of qsv_apply_ptm_batched for a square dense 64 x 64 PTM in one
  * pass: xbar = P^T ybar (per state, P = ptm_dev + b * ptm_batch_stride,
  * row-major in the state's element type) and dptm (+)= sum over orbits of

@@ -68,6 +68,10 @@ _SIGNATURES = {
     'qsv_ptm_grad': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _c_i64p,
                                     ctypes.c_int, _c_i32p, _c_i32p, _vp, ctypes.c_int, _vp,
                                     ctypes.c_int64, _vp]),
+    'qsv_ptm_grad_noalloc': (ctypes.c_int, [ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                            _c_i64p, ctypes.c_int, _c_i32p, _c_i32p, _vp,
+                                            ctypes.c_int, _vp]),
+    'qsv_ptm_grad_noalloc_scratch_bytes': (ctypes.c_int64, []),
     'qsv_emulate_ptm_grad': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int, _c_i64p,
                                             ctypes.c_int, _c_i32p, _c_i32p, _c_f64p,
                                             ctypes.c_int32, _c_i32p]),

@@ -23,6 +23,7 @@
 
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace qsv {
@@ -631,11 +632,16 @@ extern "C" int qsv_ptm_grad(int dtype, const void* x, const void* ybar, int batc
                             accumulate, workspace, workspace_bytes, nullptr, stream);
 }
 
-extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, int batch, int rank,
-                                  const int64_t* dims, int k, const int32_t* targets,
-                                  const int32_t* dims_out_t, void* dptm, int accumulate,
-                                  void* workspace, int64_t workspace_bytes, const qsv_tile* tile,
-                                  void* stream) {
+// Launches of qsv_ptm_grad_tiled and qsv_ptm_grad_noalloc: the partial
+// tiles go to `partial` (capacity `partial_bytes`).  With `shrink` the
+// planner's blocks per state are lowered until the tiles fit (the result
+// changes only in the order of the float64 partial sums); without it a
+// workspace that is too small is an error.
+static int ptm_grad_run(int dtype, const void* x, const void* ybar, int batch, int rank,
+                        const int64_t* dims, int k, const int32_t* targets,
+                        const int32_t* dims_out_t, void* dptm, int accumulate, void* partial_buf,
+                        int64_t partial_bytes, bool shrink, const qsv_tile* tile,
+                        void* stream) {
   QSV_CHECK_ARG(dtype == QSV_F32 || dtype == QSV_F64, "unknown dtype %d", dtype);
   QSV_CHECK_ARG(!tile || (tile->grad_blocks >= 0 && tile->grad_blocks <= kGradBlocks),
                 "tile: grad_blocks outside [0, %d]", kGradBlocks);
@@ -648,15 +654,23 @@ extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, in
   QSV_CHECK_ARG(!tile || tile->grad_blocks <= max_grad_blocks(p),
                 "tile: grad_blocks %d above %d for a %d x %d PTM", tile ? tile->grad_blocks : 0,
                 max_grad_blocks(p), p.dout, p.din);
-  const int nb = tile && tile->grad_blocks > 0
-                     ? tile->grad_blocks
-                     : (grad_wide(p) ? grad64_blocks(p.n_orbits, batch)
-                                     : grad_blocks(p.n_orbits, batch));
-  const int64_t need = int64_t(batch) * nb * p.din * p.dout * 8;
-  QSV_CHECK_ARG(workspace && workspace_bytes >= need, "PTM gradient needs %lld workspace bytes",
+  int nb = tile && tile->grad_blocks > 0
+               ? tile->grad_blocks
+               : (grad_wide(p) ? grad64_blocks(p.n_orbits, batch)
+                               : grad_blocks(p.n_orbits, batch));
+  const int64_t per_block = int64_t(batch) * p.din * p.dout * 8;
+  if (shrink) {
+    QSV_CHECK_ARG(per_block <= partial_bytes,
+                  "PTM gradient of %d states with a %d x %d PTM needs at least %lld bytes of "
+                  "scratch, above the %lld this entry point holds",
+                  batch, p.dout, p.din, (long long)per_block, (long long)partial_bytes);
+    nb = int(std::min<int64_t>(nb, partial_bytes / per_block));
+  }
+  const int64_t need = per_block * nb;
+  QSV_CHECK_ARG(partial_buf && partial_bytes >= need, "PTM gradient needs %lld workspace bytes",
                 (long long)need);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  double* partial = static_cast<double*>(workspace);
+  double* partial = static_cast<double*>(partial_buf);
   const dim3 grid(nb, batch);
   const bool aligned = reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
                        reinterpret_cast<uintptr_t>(ybar) % 16 == 0;
@@ -678,6 +692,85 @@ extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, in
                        static_cast<float*>(dptm), acc_flag);
   QSV_CHECK_HIP(hipGetLastError());
   return QSV_OK;
+}
+
+extern "C" int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, int batch, int rank,
+                                  const int64_t* dims, int k, const int32_t* targets,
+                                  const int32_t* dims_out_t, void* dptm, int accumulate,
+                                  void* workspace, int64_t workspace_bytes, const qsv_tile* tile,
+                                  void* stream) {
+  return ptm_grad_run(dtype, x, ybar, batch, rank, dims, k, targets, dims_out_t, dptm,
+                      accumulate, workspace, workspace_bytes, false, tile, stream);
+}
+
+// ---- zero-allocation variant ---------------------------------------------
+// The partial tiles of qsv_ptm_grad_noalloc live in this file-scope device
+// array: one copy per GPU, held until the process ends.  In host code the
+// array's name is only a stand-in for it; its device address on each GPU is
+// looked up once with hipGetSymbolAddress (on the first call with that
+// device current) and that address is what the kernels receive.  Every
+// block writes its whole partial tile before the final sum reads it, so
+// nothing carried over from an earlier call is ever read and nothing needs
+// clearing; calls that overlap on one GPU (two streams at once) race on it.
+namespace qsv {
+static __device__ __align__(256) double scratch[QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES / 8];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, which must be the
+// device of `stream`.
+static int noalloc_scratch(void* stream, double** out) {
+  int dev = -1;
+  QSV_CHECK_HIP(hipGetDevice(&dev));
+  QSV_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d outside [0, %d)", dev, kMaxDevices);
+  if (stream) {
+    hipDevice_t sdev = -1;
+    QSV_CHECK_HIP(hipStreamGetDevice(static_cast<hipStream_t>(stream), &sdev));
+    QSV_CHECK_ARG(int(sdev) == dev, "the stream belongs to device %d, device %d is current",
+                  int(sdev), dev);
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    QSV_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    QSV_CHECK_ARG(ptr != nullptr, "no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return QSV_OK;
+}
+}  // namespace qsv
+
+extern "C" int64_t qsv_ptm_grad_noalloc_scratch_bytes(void) {
+  return QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES;
+}
+
+extern "C" int qsv_ptm_grad_noalloc(int dtype, const void* x, const void* ybar, int batch,
+                                    int rank, const int64_t* dims, int k, const int32_t* targets,
+                                    const int32_t* dims_out_t, void* dptm, int accumulate,
+                                    void* stream) {
+  // argument checks first: a rejected call touches no device state
+  QSV_CHECK_ARG(dtype == QSV_F32 || dtype == QSV_F64, "unknown dtype %d", dtype);
+  QSV_CHECK_ARG(batch >= 0 && batch <= 65535, "batch %d outside [0, 65535]", batch);
+  GradPlan p;
+  const int rc = make_grad_plan(dtype_size(dtype), rank, dims, k, targets, dims_out_t, p);
+  if (rc) return rc;
+  if (batch == 0) return QSV_OK;
+  const int64_t least = int64_t(batch) * p.din * p.dout * 8;
+  QSV_CHECK_ARG(least <= QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES,
+                "PTM gradient of %d states with a %d x %d PTM needs at least %lld bytes of "
+                "scratch, above the %lld this entry point holds",
+                batch, p.dout, p.din, (long long)least,
+                (long long)QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES);
+  QSV_CHECK_ARG(x && ybar && dptm, "null pointer argument");
+  double* ptr = nullptr;
+  const int rs = noalloc_scratch(stream, &ptr);
+  if (rs) return rs;
+  return ptm_grad_run(dtype, x, ybar, batch, rank, dims, k, targets, dims_out_t, dptm,
+                      accumulate, ptr, QSV_PTM_GRAD_NOALLOC_SCRATCH_BYTES, true, nullptr,
+                      stream);
 }
 
 // Host replay of k_ptm_grad64 + k_ptm_grad_final for one state (same checks
