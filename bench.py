@@ -207,8 +207,9 @@ def bench_batched(args):
                       checkpoint_budget=budget)
     w = cb.random_weights(7)
     # the warm-up step before capture (tuned shapes) is counted: every
-    # library launch of one step, timed with events around it
-    led = Ledger(timed=True)
+    # library launch of one step, each timed alone (device idle before and
+    # after it: the two streams would otherwise overlap inside the events)
+    led = Ledger(timed=True, isolate=True)
     cb.capture(w, autotune=True, ledger=led)
     per_kernel = led.summary(HBM_PEAK_GBS)
     alg_bytes = led.total_bytes()
@@ -305,8 +306,8 @@ def bench_batched(args):
                         'note': 'alg bytes: every library launch of one step ((N_in + N_out) '
                                 's per apply, 2 N s + dP per PTM gradient, trajectories of the '
                                 'persistent kernels); achieved = bytes / graph-replay step '
-                                'time; per_kernel ms from events around each launch in one '
-                                'eager step'},
+                                'time; per_kernel ms: events around each launch of one eager '
+                                'step, the device idle before and after it'},
            'dtype': 'f32 state, f64 readout',
            'checkpoint_every': cb.k, 'kept_state_gb': round(cb.kept_bytes(cb.k) / 1e9, 1),
            'checkpoint_budget_gb': round(budget / 1e9, 1),
@@ -397,7 +398,7 @@ def bench_differentiable(args):
                          for a, b in zip(g_out, eager_out))
     del graph, g_out, eager_out
     torch.cuda.empty_cache()
-    with Ledger(timed=True) as led:
+    with Ledger(timed=True, isolate=True) as led:
         step()
     per_kernel = led.summary(HBM_PEAK_GBS)
     alg_bytes = led.total_bytes()

@@ -14,8 +14,11 @@ algorithm has to move:
 
 With ``timed=True`` each launch is also bracketed by HIP events on the
 stream it is queued on, so one instrumented eager step gives every family's
-GPU time and rate.  Off (the default) it costs one global lookup per launch.
-No counterpart in the reference.
+GPU time and rate.  Work on two streams overlaps, and an event pair then
+also spans the other stream's kernels; ``isolate=True`` waits for the whole
+device before and after every counted launch, so each one is timed alone
+(as a kernel trace times it).  Off (the default) the ledger costs one global
+lookup per launch.  No counterpart in the reference.
 """
 import torch
 
@@ -40,6 +43,8 @@ KERNELS = {
 }
 
 _ACTIVE = None
+# GPU clock cycles of the spin queued before an isolated launch (~100 us)
+_SPIN_CYCLES = 200000
 
 
 def active():
@@ -48,8 +53,9 @@ def active():
 
 
 class Ledger:
-    def __init__(self, timed=True):
+    def __init__(self, timed=True, isolate=False):
         self.timed = timed
+        self.isolate = isolate
         self.rows = {}        # family -> [launches, bytes, [(start, end) events]]
 
     def __enter__(self):
@@ -67,6 +73,12 @@ class Ledger:
         """Call right before a launch; returns its start event (or None)."""
         if not self.timed:
             return None
+        if self.isolate:
+            torch.cuda.synchronize()
+            # keep the stream busy while the host queues the event and the
+            # launch, so the start event fires right before the kernel runs
+            # (not when the host gets to the launch)
+            torch.cuda._sleep(_SPIN_CYCLES)
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
         return ev
@@ -80,6 +92,8 @@ class Ledger:
             ev = torch.cuda.Event(enable_timing=True)
             ev.record()
             row[2].append((start, ev))
+            if self.isolate:
+                torch.cuda.synchronize()
 
     def total_bytes(self):
         return sum(r[1] for r in self.rows.values())

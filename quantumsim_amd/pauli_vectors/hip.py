@@ -290,6 +290,17 @@ class PauliVectorHIP(PauliVectorBase):
             if self._shape[q] != d:
                 raise ValueError('PTM input dimension {} does not match dimension {} of '
                                  'qubit {}'.format(d, self._shape[q], q))
+        if not on_device and math.prod(dims_in_t) in (64, 256) and dims_in_t == dims_out_t:
+            # dense 3-/4-qubit PTMs from the host: staged through pinned
+            # memory and copied asynchronously, then applied as a device PTM
+            # (the MFMA kernels gather or pack it on the device), so the call
+            # neither blocks on a pageable copy nor copies a host table
+            # (torch's pinned allocator keeps the staging block until the
+            # copy has run)
+            host_np = np.ascontiguousarray(
+                ptm, dtype=np.float64 if self._dtype == torch.float64 else np.float32)
+            ptm = torch.from_numpy(host_np).pin_memory().to(self._device, non_blocking=True)
+            on_device = True
         if on_device:
             ptm = ptm.detach().to(self._dtype).contiguous()
         else:

@@ -117,6 +117,15 @@ class FinalizedCircuit:
         self.operation(state.pauli_vector, *indices)
 
 
+def _fixed_dtype(dtype):
+    """PauliVectorHIP with a default element type (State constructs the
+    class with bases only)."""
+    class HIPVector(qa.PauliVectorHIP):
+        def __init__(self, bases_, pv=None, *, force=False, dtype=dtype, device=None):
+            super().__init__(bases_, pv, force=force, dtype=dtype, device=device)
+    return HIPVector
+
+
 def _kraus_set():
     """(name, Kraus operators, number of qubits, basis of the operation)."""
     rng = np.random.default_rng(17)
@@ -163,10 +172,7 @@ def test_state_and_circuit_contract_against_oracle(dtype, tol):
     hip_ops = _ops(bases, kraus_to_ptm, lambda p, bi, bo: PTMOp(p, bi, bo))
     orc_ops = _ops(ob, oa.kraus_to_ptm, _OraclePTMOp)
 
-    hip_cls_t = type('HIPVector', (qa.PauliVectorHIP,), {
-        '__init__': lambda self, bs, pv=None, force=False:
-            qa.PauliVectorHIP.__init__(self, bs, pv, force=force, dtype=dtype)})
-    s_hip = State(state_qubits, pauli_vector_class=hip_cls_t)
+    s_hip = State(state_qubits, pauli_vector_class=_fixed_dtype(dtype))
     s_orc = State(state_qubits, pauli_vector_class=PauliVectorNumpy, basis_lib=ob)
     # every qubit starts in its one-element subbasis: one entry in all
     assert s_hip.pauli_vector.dim_pauli == (1, 1, 1, 1)
@@ -194,9 +200,9 @@ def test_state_and_circuit_contract_against_oracle(dtype, tol):
     # names out of state order with unequal dimensions: the data axes stay
     # ascending while the bases follow the names, so both backends refuse
     # the shape (numpy.py:86-95)
-    for s in (s_hip, s_orc):
+    for st in (s_hip, s_orc):
         with pytest.raises(ValueError):
-            s.partial_trace('B', 'D')
+            st.partial_trace('B', 'D')
     # the State's class gate and the circuit's qubit lookup
     with pytest.raises(ValueError):
         State(['A'], pauli_vector_class=dict)
@@ -204,7 +210,6 @@ def test_state_and_circuit_contract_against_oracle(dtype, tol):
         FinalizedCircuit(Chain([(hip_ops['rx'], (0,))]), ['Z']) @ s_hip
     with pytest.raises(ValueError):
         Chain([(hip_ops['cphase'], (0, 1))])(vh, 0)
-    del hip_cls
 
 
 def test_measurement_reduction_through_the_caller():
@@ -234,8 +239,9 @@ def test_measurement_reduction_through_the_caller():
     s_orc = State(['q0', 'q1'], pauli_vector_class=PauliVectorNumpy, basis_lib=ob)
     for i, ((oh, ix), (oo, _)) in enumerate(zip(hip, orc)):
         pre = s_hip.pauli_vector.to_pv()
-        FinalizedCircuit(Chain([(oh, ix)]), ['q%d' % q for q in ix]) @ s_hip
-        FinalizedCircuit(Chain([(oo, ix)]), ['q%d' % q for q in ix]) @ s_orc
+        local = tuple(range(len(ix)))            # unit indices are the circuit's own
+        FinalizedCircuit(Chain([(oh, local)]), ['q%d' % q for q in ix]) @ s_hip
+        FinalizedCircuit(Chain([(oo, local)]), ['q%d' % q for q in ix]) @ s_orc
         if i == 2:                       # the projection is a gather: bit-exact
             assert s_hip.pauli_vector.dim_pauli == (4, 2)
             assert np.array_equal(s_hip.pauli_vector.to_pv(), pre[:, :2])

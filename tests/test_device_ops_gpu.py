@@ -64,6 +64,37 @@ def test_device_ptm_bitwise_equals_host_ptm(n, targets, dout, dtype):
     assert torch.equal(a.to_torch(), b.to_torch())
 
 
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+@pytest.mark.parametrize('n,targets', [(6, (1, 2, 3)), (7, (0, 3, 6)), (6, (5, 4, 3)),
+                                       (6, (2, 3, 4, 5)), (6, (0, 2, 4, 5))])
+def test_dense_host_ptm_c_abi_equals_staged_path(n, targets, dtype):
+    """Dense 64 x 64 / 256 x 256 host PTMs: PauliVectorHIP stages them
+    through pinned memory and applies them as device PTMs; the C ABI's
+    host-PTM entry point (qsv_apply_ptm: fragment-ordered copy into the
+    workspace) gives the same bits."""
+    from quantumsim_amd import _native as nat
+    rng = np.random.default_rng(n * 7 + len(targets))
+    k = len(targets)
+    P = np.ascontiguousarray(rng.standard_normal((4,) * (2 * k)))
+    pv = rng.standard_normal((4,) * n)
+    print('host PTM via C ABI', n, targets, dtype, flush=True)
+    v = qa.PauliVectorHIP([general(2)] * n, pv, force=True, dtype=dtype)
+    v.apply_ptm(P, *targets)
+    x = torch.tensor(pv, dtype=dtype, device=DEV).reshape(-1)
+    y = torch.empty_like(x)
+    L = nat.lib()
+    code = nat.dtype_code(dtype)
+    ws_bytes = nat.check(L.qsv_apply_ptm_workspace_bytes(code, k, nat.i32([4] * k),
+                                                         nat.i32([4] * k)), 'ws')
+    ws = torch.empty(max(ws_bytes, 8), dtype=torch.uint8, device=DEV)
+    rc = L.qsv_apply_ptm(code, x.data_ptr(), y.data_ptr(), n, nat.i64([4] * n), k,
+                         nat.i32(targets), nat.i32([4] * k), P.ctypes.data_as(nat._c_f64p),
+                         ws.data_ptr(), ws_bytes, torch.cuda.current_stream().cuda_stream)
+    nat.check(rc, 'apply_ptm')
+    torch.cuda.synchronize()
+    assert torch.equal(v.to_torch().reshape(-1), y)
+
+
 @pytest.mark.parametrize('targets', [(0, 1, 2), (9, 10, 11), (0, 5, 11), (0, 1)])
 def test_device_ptm_padded_layout(targets):
     """12 qubits float64 (128 MiB): the outer axes' strides are padded."""

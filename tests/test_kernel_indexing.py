@@ -325,7 +325,8 @@ def test_grad64_load_modes():
     target, scalar loads otherwise."""
     rng = np.random.default_rng(4)
     x = rng.standard_normal([4] * 5)
-    for dtype, vec in ((nat.QSV_F64, 2), (nat.QSV_F32, 4)):
+    # float64 orbit vectors go to the LDS-staged kernel (512)
+    for dtype, vec in ((nat.QSV_F64, 2 | 512), (nat.QSV_F32, 4)):
         assert emulate_grad(dtype, x, x, (1, 2, 3), [4, 4, 4])[1] == vec
         assert emulate_grad(dtype, x, x, (2, 3, 4), [4, 4, 4])[1] == 1 | 256
         assert emulate_grad(dtype, x, x, (4, 2, 3), [4, 4, 4])[1] == 1
