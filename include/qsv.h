@@ -74,6 +74,25 @@ typedef struct qsv_gate {
   int64_t ptm;
 } qsv_gate;
 
+/* One block position of a brickwork window (two consecutive layers of a
+ * layered noisy circuit), for qsv_window_blocks: the block's qubits
+ * (ascending; qubits[0] is the most significant PTM digit), its output
+ * array (element offset of a [n_windows][batch][D][D] array, D = 4^nq) and
+ * up to four factors applied in order.  Factor f: kind 1 = a lone qubit of
+ * layer f_layer (PTM D_q R_q on slot f_slot0), kind 2 = a bond of layer
+ * f_layer on slots (f_slot0, f_slot1) (PTM (D_a (x) D_b) CZ (R_a (x) R_b)). */
+typedef struct qsv_wblock {
+  int32_t nq;
+  int32_t qubits[3];
+  int32_t nf;
+  int32_t reserved;
+  int64_t out_off;
+  int32_t f_kind[4];
+  int32_t f_layer[4];
+  int32_t f_slot0[4];
+  int32_t f_slot1[4];
+} qsv_wblock;
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* qsv_last_error(void);
 
@@ -188,6 +207,44 @@ int qsv_circuits_adjoint(int dtype, int n_circuits, const qsv_circuit* circs_dev
                          const qsv_gate* gates_dev, const void* ptms, const void* traj,
                          const double* dpops, double* dptms, int max_qubits, void* stream);
 
+/* Dense blocks of every window of `batch` layered circuits of n_qubits
+ * qubits, one launch: for window w, circuit b and position p of the table,
+ * out[p.out_off + ((w * batch + b) * D + o) * D + i] = the product of the
+ * position's factors built from
+ *   rot  [batch][layers][n_qubits][4][4]  one-qubit PTM of layer 2w + l,
+ *   damp [batch][n_qubits][4][4]          damping PTM of each qubit,
+ *   cz   [16][16]                          the CZ PTM,
+ * with the layer-1 factors replaced by the identity where keep[b * n_windows
+ * + w] == 0 (a circuit whose depth ends inside the window).  float64
+ * arithmetic, output in the state's element type.  table_host (validated on
+ * the host) and table_dev hold the same n_pos entries.  No counterpart in
+ * the reference (which composes PTMs on the host per operation); it replaces
+ * the block composition of the batched path. */
+int qsv_window_blocks(int dtype, int n_pos, const qsv_wblock* table_host,
+                      const qsv_wblock* table_dev, int batch, int n_windows, int n_qubits,
+                      int layers, const double* rot, const double* damp, const uint8_t* keep,
+                      const double* cz, void* out, int64_t out_elems, void* stream);
+
+/* Backward of qsv_window_blocks: from dL/dout (same layout, state type)
+ * writes every entry of drot [batch][layers][n_qubits][4][4] and of
+ * ddamp_part [batch][n_windows][2][n_qubits][4][4] (the damping gradient of
+ * each window and layer; the caller sums them); float64. */
+int qsv_window_blocks_backward(int dtype, int n_pos, const qsv_wblock* table_host,
+                               const qsv_wblock* table_dev, int batch, int n_windows,
+                               int n_qubits, int layers, const double* rot, const double* damp,
+                               const uint8_t* keep, const double* cz, const void* dout,
+                               int64_t out_elems, double* drot, double* ddamp_part,
+                               void* stream);
+
+/* CPU replay of both window-block kernels on host float64 buffers (the same
+ * per-thread phases, every index checked); dout == NULL replays the forward
+ * kernel only.  No counterpart in the reference. */
+int qsv_emulate_window_blocks(int n_pos, const qsv_wblock* table, int batch, int n_windows,
+                              int n_qubits, int layers, const double* rot, const double* damp,
+                              const uint8_t* keep, const double* cz, double* out,
+                              int64_t out_elems, const double* dout, double* drot,
+                              double* ddamp_part);
+
 /* Bytes of device workspace qsv_ptm_grad / qsv_ptm_grad_tiled need (enough
  * for any launch shape). */
 int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64_t* dims, int k,
@@ -264,8 +321,10 @@ int qsv_ptm_grad_tiled(int dtype, const void* x, const void* ybar, int batch, in
  * float64 [dout * din].  grad_blocks: blocks per state, as the qsv_tile field (0:
  * automatic for a batch of one).  *vn_out (may be NULL) = orbits per lane
  * vector the GPU call would use for `dtype`, plus 256 when 64 x 64 PTMs
- * load runs of four entries as 16-byte vectors (innermost axis a target).  No counterpart in the reference; it lets
- * CPU tests pin the gradient kernel's indexing before GPU use. */
+ * load runs of four entries as 16-byte vectors (innermost axis a target),
+ * plus 512 when the LDS-staged 64 x 64 kernel runs (float64 states with
+ * orbit vectors).  No counterpart in the reference; it lets CPU tests pin
+ * the gradient kernel's indexing before GPU use. */
 int qsv_emulate_ptm_grad(int dtype, const double* x, const double* ybar, int rank,
                          const int64_t* dims, int k, const int32_t* targets,
                          const int32_t* dims_out_t, double* dptm, int32_t grad_blocks,

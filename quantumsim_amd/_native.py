@@ -83,6 +83,17 @@ _SIGNATURES = {
                                                               _c_f64p, ctypes.c_int, _c_i64p,
                                                               ctypes.c_int, _c_i32p, _c_f64p,
                                                               _c_f64p, ctypes.c_int32]),
+    'qsv_window_blocks': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                         _vp, _vp, ctypes.c_int64, _vp]),
+    'qsv_window_blocks_backward': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                                  ctypes.c_int64, _vp, _vp, _vp]),
+    'qsv_emulate_window_blocks': (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, _c_f64p, _c_f64p,
+                                                 _vp, _c_f64p, _c_f64p, ctypes.c_int64, _c_f64p,
+                                                 _c_f64p, _c_f64p]),
     'qsv_circuits_max_qubits': (ctypes.c_int, [ctypes.c_int]),
     'qsv_circuits_lds_bytes': (ctypes.c_int64, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     'qsv_circuits_forward': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,

@@ -41,12 +41,13 @@ import numpy as np
 import torch
 
 from . import persistent
-from .autograd import Gate, run_schedule, diagonal, shared_constant
+from .autograd import Gate, run_schedule, diagonal, shared_constant, cz_ptm
 from .autotune import tuning
 from .bases import general
 from .fusion import fuse_batched, fuse_plan
 from .workloads import (rotation_ptms, damping_ptms, fused_layer_gates, layer_targets,
                         brickwork_window_plan)
+from .windows import WindowPlan, window_blocks
 
 __all__ = ['CircuitBatch', 'plain_forward_backward', 'choose_checkpoint_interval']
 
@@ -93,7 +94,8 @@ class CircuitBatch:
 
     def __init__(self, circuits, dtype=torch.float32, device=None, checkpoint_every=None,
                  requires_grad=True, persistent_kernel=True, fuse_qudits=3,
-                 checkpoint_budget=None, renormalize=True, overlap_narrow=True):
+                 checkpoint_budget=None, renormalize=True, overlap_narrow=True,
+                 block_kernel=True):
         self.device = device or torch.device('cuda', torch.cuda.current_device())
         self.dtype = dtype
         self.renormalize = renormalize
@@ -103,6 +105,9 @@ class CircuitBatch:
         # the MFMA kernels; 2 or less: the layer's own gates, one at a time)
         self.fuse_qudits = int(fuse_qudits or 0)
         self.window = 2 if self.fuse_qudits > 2 else 1
+        # dense 3-qubit window blocks built by one kernel launch per group
+        # (windows.window_blocks) instead of torch products
+        self.block_kernel = bool(block_kernel)
         self._consts = {}
         self.circuits = circuits
         # one stream (default priority) for parameters, eager steps and the
@@ -261,6 +266,21 @@ class CircuitBatch:
             ang = torch.cat([ang, torch.zeros_like(ang[:, :1])], 1)
         rot = rotation_ptms(ang)                                  # [B, 2 nw, n, 4, 4]
         damp = damping_ptms(g.t1, g.t2)                           # [B, n, 4, 4]
+        if self.block_kernel and self.fuse_qudits == 3:
+            # every block of every window and circuit in one launch (and one
+            # for the backward): qsv_window_blocks
+            plan = self._consts.get(('wplan', id(g)))
+            if plan is None:
+                plan = WindowPlan(n, nw, B, self.device)
+                shared_constant(plan.table_dev)
+                self._consts[('wplan', id(g))] = plan
+            keep8 = self._cached(('keep8', id(g)), lambda: torch.tensor(
+                [[g.layers[b] > 2 * w + 1 for w in range(nw)] for b in range(B)],
+                dtype=torch.uint8, device=self.device))
+            cz = self._cached(('cz16',), lambda: cz_ptm(self.device).reshape(16, 16)
+                              .contiguous())
+            blocks = window_blocks(rot, damp, keep8, plan, self.dtype, cz)
+            return [(m, qs) for m, (qs, _, _) in zip(blocks, plan.positions)]
         damp_w = damp[:, None].expand(B, nw, n, 4, 4).reshape(B * nw, n, 4, 4)
         keep = self._cached(('keep', id(g)), lambda: torch.tensor(
             [g.layers[b] > 2 * w + 1 for b in range(B) for w in range(nw)],

@@ -1,0 +1,66 @@
+"""qsv_window_blocks on the GPU: blocks and gradients equal the torch
+composition of the same windows (float64 to 1e-14), and a CircuitBatch
+built on them gives the populations and parameter gradients of the torch
+path (float64 1e-12, float32 1e-6), in eager steps and graph replays."""
+import numpy as np
+import pytest
+import torch
+
+from quantumsim_amd.windows import WindowPlan, window_blocks
+from quantumsim_amd.workloads import rotation_ptms, damping_ptms, random_circuit_batch
+from quantumsim_amd.autograd import cz_ptm
+from quantumsim_amd.batch import CircuitBatch
+from test_window_blocks import torch_blocks, random_factors
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda'
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+@pytest.mark.parametrize('n', [4, 5, 8, 11])
+def test_kernel_blocks_and_gradients_match_torch(n, dtype):
+    rng = np.random.default_rng(10 + n)
+    B, nw = 5, 3
+    ang, t1, t2, keep = random_factors(n, B, nw, rng, DEV)
+    rot, damp = rotation_ptms(ang), damping_ptms(t1, t2)
+    print('window blocks', n, dtype, flush=True)
+    ref = torch_blocks(rot, damp, keep, n, nw)
+    plan = WindowPlan(n, nw, B, torch.device(DEV))
+    cz = cz_ptm(torch.device(DEV)).reshape(16, 16).contiguous()
+    got = window_blocks(rot, damp, keep, plan, dtype, cz)
+    tol = 1e-14 if dtype == torch.float64 else 1e-6
+    for a, r in zip(got, ref):
+        assert a.dtype == dtype and a.is_contiguous()
+        assert (a.double() - r.double()).abs().max().item() < tol
+    ups = [torch.randn(r.shape, dtype=torch.float64, device=DEV) for r in ref]
+    g_ref = torch.autograd.grad(sum((r * u).sum() for r, u in zip(ref, ups)), [ang, t1, t2])
+    g_got = torch.autograd.grad(sum((a.double() * u).sum() for a, u in zip(got, ups)),
+                                [ang, t1, t2])
+    for a, r in zip(g_got, g_ref):
+        scale = max(1.0, r.abs().max().item())
+        assert (a - r).abs().max().item() < (1e-13 if dtype == torch.float64 else 1e-5) * scale
+
+
+@pytest.mark.parametrize('dtype,tol', [(torch.float64, 1e-12), (torch.float32, 1e-6)])
+def test_circuit_batch_with_block_kernel_matches_torch_blocks(dtype, tol):
+    circuits = random_circuit_batch(24, seed=31, n_range=(4, 11), layer_range=(3, 15))
+    kb = CircuitBatch(circuits, dtype=dtype, checkpoint_every=2)
+    tb = CircuitBatch(circuits, dtype=dtype, checkpoint_every=2, block_kernel=False)
+    w = kb.random_weights(4)
+    _, pk, gk = kb.step(w)
+    _, pt, gt = tb.step(w)
+    for a, b in zip(pk, pt):
+        assert (a - b).abs().max().item() < tol
+    for a, b in zip(gk, gt):
+        scale = max(1e-30, b.abs().max().item())
+        assert (a - b).abs().max().item() < (1e-10 if dtype == torch.float64 else 1e-4) * scale
+    # the kernel path inside a captured graph
+    kb.capture(w, autotune=False)
+    for rep in range(2):
+        torch.randn(1000, device=DEV).sum().item()
+        _, pg, gg = kb.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(pg, pk):
+            assert torch.equal(a, b), rep
+        for a, b in zip(gg, gk):
+            assert torch.equal(a, b), rep
