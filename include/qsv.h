@@ -137,7 +137,9 @@ int qsv_apply_ptm_strided(int dtype, const void* in, void* out, int rank,
 /* Same contraction with the PTM already on the device, C-order
  * (out..., in...), in the state's element type (used when PTMs are built
  * on the device, e.g. from trainable parameters, and under graph capture).
- * `workspace` holds the offset tables (qsv_apply_ptm_workspace_bytes). */
+ * `workspace` is used only by dense 256 x 256 PTMs (D * D elements); PTMs
+ * of any other size up to 1024 entries a side need none (the table kernel
+ * decodes its offsets on the device). */
 int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
                       const int64_t* dims, int k, const int32_t* targets,
                       const int32_t* dims_out_t, const void* ptm_dev,
@@ -158,9 +160,12 @@ int qsv_apply_ptm_dev_strided(int dtype, const void* in, void* out, int rank,
                               const int32_t* dims_out_t, const void* ptm_dev,
                               void* workspace, int64_t workspace_bytes, void* stream);
 
-/* Batched form of qsv_apply_ptm_dev for PTMs up to 16 x 16 and dense
- * 64 x 64 PTMs (three-qubit blocks, MFMA kernel, each state's row-major PTM
- * gathered into fragment order inside the kernel): `batch` states
+/* Batched form of qsv_apply_ptm_dev for PTMs of any size up to 1024
+ * entries a side: inline kernels up to 16 x 16, dense 64 x 64 PTMs
+ * (three-qubit blocks) on the MFMA kernel with each state's row-major PTM
+ * gathered into fragment order inside the kernel, any other size (qutrit
+ * pairs 81 x 81, qutrit + qubit 36 x 36, ...) on the table kernel with the
+ * offsets decoded on the device; no workspace either way: `batch` states
  * of identical shape, stored back to back (in: batch x N_in, out: batch x
  * N_out), each contracted with its own device PTM
  * ptm_dev + b * ptm_batch_stride (elements of the state's type; a stride of
@@ -251,7 +256,9 @@ int64_t qsv_ptm_grad_workspace_bytes(int batch, int rank, const int64_t* dims, i
                                      const int32_t* targets, const int32_t* dims_out_t);
 
 /* Gradient of a PTM application with respect to the PTM, for `batch` states
- * back to back (the layout of qsv_apply_ptm_batched), PTMs up to 64 x 64:
+ * back to back (the layout of qsv_apply_ptm_batched), PTMs up to 256
+ * entries a side and 8192 entries (MFMA kernels up to 64 x 64, a float64
+ * LDS kernel above: qutrit pairs 81 x 81, ...):
  *   dptm[b, o, i] (+)= sum over orbits of ybar[b, .., o, ..] * x[b, .., i, ..]
  * x: the states the PTM was applied to; ybar: the loss gradient with
  * respect to the results.  Accumulated in float64 in a fixed order

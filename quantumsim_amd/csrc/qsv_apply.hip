@@ -149,6 +149,75 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ------------------------------------------------------- table, batched
+// The table kernel for PTMs in device memory of any size (qutrit pairs 81 x
+// 81, qubit-qutrit 36 x 36, ...), batched (grid.y = state, each state's PTM
+// pstride elements apart, 0 = shared), with the entry offsets decoded by
+// every block into LDS from the targets' dims and strides: no host table,
+// no copy, so it is safe under graph capture.
+struct TableGeo {
+  int32_t k;
+  int32_t din_t[QSV_MAX_TARGETS], dout_t[QSV_MAX_TARGETS];
+  int64_t sin_t[QSV_MAX_TARGETS], sout_t[QSV_MAX_TARGETS];
+};
+constexpr int kTableMaxD = 1024;  // entries per side the LDS offsets hold
+
+// Offset of PTM index e on one side: mixed radix over the targets in PTM
+// axis order, the last target fastest.
+__host__ __device__ inline int64_t table_offset(int k, const int32_t* d, const int64_t* st,
+                                                int e) {
+  int64_t off = 0;
+  for (int j = k - 1; j >= 0; --j) {
+    off += int64_t(e % d[j]) * st[j];
+    e /= d[j];
+  }
+  return off;
+}
+
+template <typename T, int CH, int VEC>
+__global__ void __launch_bounds__(kBlock)
+    k_apply_table_b(const T* __restrict__ in, T* __restrict__ out, const Geom g,
+                    const TableGeo tg, const T* __restrict__ ptm, const int64_t pstride) {
+  typedef typename VT<T, VEC>::t V;
+  __shared__ int64_t so_in[kTableMaxD], so_out[kTableMaxD];
+  for (int e = threadIdx.x; e < g.din; e += kBlock)
+    so_in[e] = table_offset(tg.k, tg.din_t, tg.sin_t, e);
+  for (int e = threadIdx.x; e < g.dout; e += kBlock)
+    so_out[e] = table_offset(tg.k, tg.dout_t, tg.sout_t, e);
+  __syncthreads();
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= g.n_threads) return;
+  const uint32_t b = blockIdx.y;
+  in += int64_t(b) * g.bs_in;
+  out += int64_t(b) * g.bs_out;
+  ptm += int64_t(b) * pstride;
+  int64_t bin, bout;
+  orbit_base(g, t * VEC, bin, bout);
+  for (int oc = 0; oc < g.dout; oc += CH) {
+    V acc[CH];
+#pragma unroll
+    for (int o = 0; o < CH; ++o) acc[o] = V(0);
+    for (int ic = 0; ic < g.din; ic += CH) {
+      V v[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        v[i] = ic + i < g.din ? *reinterpret_cast<const V*>(in + bin + so_in[ic + i]) : V(0);
+#pragma unroll
+      for (int o = 0; o < CH; ++o) {
+        if (oc + o < g.dout) {
+          const T* prow = ptm + int64_t(oc + o) * g.din + ic;
+#pragma unroll
+          for (int i = 0; i < CH; ++i)
+            if (ic + i < g.din) acc[o] += prow[i] * v[i];
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < CH; ++o)
+      if (oc + o < g.dout) *reinterpret_cast<V*>(out + bout + so_out[oc + o]) = acc[o];
+  }
+}
+
 
 // ------------------------------------------------------------ LDS tiled
 // For targets near the innermost axis the orbit entries of neighbouring
@@ -1003,19 +1072,49 @@ static int run_host_table(const Plan& p, const double* ptm_f64, void* workspace,
   return run_table<T>(p, op, in, out, s);
 }
 
-template <typename T>
-static int run_dev_table(const Plan& p, const void* ptm_dev, void* workspace, const void* in,
-                         void* out, hipStream_t s) {
-  const int64_t dinp = round_up(p.din, 16), doutp = round_up(p.dout, 16);
-  std::vector<int64_t> host(dinp + doutp, 0);
-  for (int i = 0; i < p.din; ++i) host[i] = p.off_in[i];
-  for (int o = 0; o < p.dout; ++o) host[dinp + o] = p.off_out[o];
-  QSV_CHECK_HIP(
-      hipMemcpyAsync(workspace, host.data(), host.size() * 8, hipMemcpyHostToDevice, s));
-  const int64_t* wb = static_cast<const int64_t*>(workspace);
-  TableOp<T> op{wb, wb + dinp, static_cast<const T*>(ptm_dev), p.din, 0};
-  return run_table<T>(p, op, in, out, s);
+
+// TableGeo of a plan, checked against the plan's own offsets.
+static int make_table_geo(const Plan& p, TableGeo& tg) {
+  QSV_CHECK_ARG(p.din <= kTableMaxD && p.dout <= kTableMaxD,
+                "PTM of %d x %d: the table kernel takes up to %d entries per side", p.dout,
+                p.din, kTableMaxD);
+  memset(&tg, 0, sizeof(tg));
+  tg.k = p.k;
+  for (int j = 0; j < p.k; ++j) {
+    const int q = p.targets[j];
+    tg.din_t[j] = int32_t(p.dims_in[q]);
+    tg.dout_t[j] = int32_t(p.dims_out[q]);
+    tg.sin_t[j] = p.sin[q];
+    tg.sout_t[j] = p.sout[q];
+  }
+  for (int e = 0; e < p.din; ++e)
+    QSV_CHECK_ARG(table_offset(tg.k, tg.din_t, tg.sin_t, e) == p.off_in[e],
+                  "internal: table offsets disagree (input %d)", e);
+  for (int e = 0; e < p.dout; ++e)
+    QSV_CHECK_ARG(table_offset(tg.k, tg.dout_t, tg.sout_t, e) == p.off_out[e],
+                  "internal: table offsets disagree (output %d)", e);
+  return QSV_OK;
 }
+
+template <typename T>
+static int run_table_b(const Plan& p, const void* ptm_dev, int64_t pstride, uint32_t batch,
+                       const void* in, void* out, hipStream_t s) {
+  TableGeo tg;
+  const int rc = make_table_geo(p, tg);
+  if (rc) return rc;
+  if (p.g.n_threads == 0 || batch == 0) return QSV_OK;
+  const dim3 grid(ceil_div_u32(p.g.n_threads, kBlock), batch);
+  const T* pt = static_cast<const T*>(ptm_dev);
+  if (p.vec == 16 / int(sizeof(T)))
+    hipLaunchKernelGGL((k_apply_table_b<T, 16, 16 / sizeof(T)>), grid, dim3(kBlock), 0, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), p.g, tg, pt, pstride);
+  else
+    hipLaunchKernelGGL((k_apply_table_b<T, 16, 1>), grid, dim3(kBlock), 0, s,
+                       static_cast<const T*>(in), static_cast<T*>(out), p.g, tg, pt, pstride);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
+}
+
 
 }  // namespace qsv
 
@@ -1108,12 +1207,12 @@ extern "C" int qsv_apply_ptm_dev_strided(int dtype, const void* in, void* out, i
     if (rp) return rp;
     return launch_fused(dtype, fp, p.g, p.nt, in, out, workspace, s);
   }
-  const int64_t need = table_bytes(dtype, p.din, p.dout, false);
-  QSV_CHECK_ARG(workspace && workspace_bytes >= need,
-                "offset tables of a %d x %d PTM need a workspace of %lld bytes", p.dout,
-                p.din, (long long)need);
-  return dtype == QSV_F64 ? run_dev_table<double>(p, ptm_dev, workspace, in, out, s)
-                          : run_dev_table<float>(p, ptm_dev, workspace, in, out, s);
+  // any other size: the table kernel with its offsets decoded on the device
+  // (no host table, no copy)
+  (void)workspace;
+  (void)workspace_bytes;
+  return dtype == QSV_F64 ? run_table_b<double>(p, ptm_dev, 0, 1, in, out, s)
+                          : run_table_b<float>(p, ptm_dev, 0, 1, in, out, s);
 }
 
 extern "C" int qsv_apply_ptm_dev(int dtype, const void* in, void* out, int rank,
@@ -1427,9 +1526,13 @@ extern "C" int qsv_apply_ptm_batched_tiled(int dtype, const void* in, void* out,
       return launch_fused64_batched(dtype, fp, p.g, p.nt, in, out, ptm_dev, ptm_batch_stride,
                                     uint32_t(batch), s);
     }
-    set_error("batched apply supports PTMs up to %d x %d and dense 64 x 64, got %d x %d",
-              kInline, kInline, p.dout, p.din);
-    return QSV_ENOTSUP;
+    // any other size (qutrit pairs, qubit-qutrit pairs, ...): the table
+    // kernel with the offsets decoded on the device
+    QSV_CHECK_ARG(!tile || tile->kernel == QSV_KERNEL_AUTO,
+                  "%d x %d PTMs take no launch shape", p.dout, p.din);
+    return dtype == QSV_F64
+               ? run_table_b<double>(p, ptm_dev, ptm_batch_stride, uint32_t(batch), in, out, s)
+               : run_table_b<float>(p, ptm_dev, ptm_batch_stride, uint32_t(batch), in, out, s);
   }
   p.batch = uint32_t(batch);
   const bool small = std::max(p.din, p.dout) <= 4;
@@ -1462,8 +1565,14 @@ extern "C" int qsv_emulate_apply_ptm_tiled(int dtype, const double* in, double* 
       if (path_out) *path_out = 4 | (fp.vi << 4) | (fp.vo << 8) | (fp.vn << 12);
       return emulate_fused(p, es, ptm_f64, in, out, 2);
     }
-    set_error("batched apply supports PTMs up to %d x %d and dense 64 x 64", kInline, kInline);
-    return QSV_ENOTSUP;
+    // the batched table kernel: offsets as the kernel decodes them
+    QSV_CHECK_ARG(!tile || tile->kernel == QSV_KERNEL_AUTO,
+                  "%d x %d PTMs take no launch shape", p.dout, p.din);
+    TableGeo tg;
+    const int rt = make_table_geo(p, tg);
+    if (rt) return rt;
+    if (path_out) *path_out = 3;
+    return emulate_direct(p, ptm_f64, in, out);
   }
   LdsPlan lp;
   if (make_lds_plan(p, es, lp, tile)) {

@@ -208,36 +208,57 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t ntiles = (n_orbits + COLS - 1) / COLS;
   const int64_t li = fo.in_l[j], lo = fo.out_l[j];
 
-  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint32_t m = tile * COLS + 16 * w + n;
-    int64_t bin, bout;
-    orbit_base(g, m < n_orbits ? m : n_orbits - 1, bin, bout);
-    const T* pin = in + bin + li;
-
-    Acc acc[MB];
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) acc[mb] = Acc(0);
-    T ar[PER], bc[CS], bn[CS];
+  // chunk 0 of the PTM and the first B fragments of a tile are loaded
+  // during the previous tile's last chunk (NCH is even, so the prefetched
+  // chunk 0 lands in As[0], where chunk 0 is read)
+  static_assert(NCH % 2 == 0, "chunk 0 must come back to buffer 0");
+  auto tile_base = [&](uint32_t t, uint32_t& mm, int64_t& bo) -> const T* {
+    mm = t * COLS + 16 * w + n;
+    int64_t bi;
+    orbit_base(g, mm < n_orbits ? mm : n_orbits - 1, bi, bo);
+    return in + bi + li;
+  };
+  T ar[PER], bc[CS], bn[CS];
+  uint32_t m = 0;
+  int64_t bout = 0;
+  const T* pin = in;
+  if (blockIdx.x < ntiles) {
+    pin = tile_base(blockIdx.x, m, bout);
 #pragma unroll
     for (int i = 0; i < PER; ++i) ar[i] = pf[i * kBlock + tid];
 #pragma unroll
     for (int uu = 0; uu < CS / VI; ++uu) load_vec<T, VI, NT>(pin + fo.in_u[uu], &bn[uu * VI]);
 #pragma unroll
     for (int i = 0; i < PER; ++i) As[0][i * kBlock + tid] = ar[i];
-    __syncthreads();
+  }
+  __syncthreads();
+
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // the next tile (this one again past the last: loaded, never used)
+    const uint32_t tn = tile + gridDim.x < ntiles ? tile + gridDim.x : tile;
+    uint32_t mn;
+    int64_t boutn;
+    const T* pinn = tile_base(tn, mn, boutn);
+
+    Acc acc[MB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) acc[mb] = Acc(0);
 
 #pragma unroll 1
     for (int c = 0; c < NCH; ++c) {
 #pragma unroll
       for (int ss = 0; ss < CS; ++ss) bc[ss] = bn[ss];
-      // next chunk (the last iteration re-reads its own: no branch)
-      const int cn = c + 1 < NCH ? c + 1 : c;
+      // next chunk of this tile, or chunk 0 and the first B fragments of
+      // the next tile
+      const bool last = c + 1 == NCH;
+      const int cn = last ? 0 : c + 1;
       const T* src = pf + int64_t(cn) * CHUNK + tid;
+      const T* pb = last ? pinn : pin;
 #pragma unroll
       for (int i = 0; i < PER; ++i) ar[i] = src[i * kBlock];
 #pragma unroll
       for (int uu = 0; uu < CS / VI; ++uu)
-        load_vec<T, VI, NT>(pin + fo.in_u[cn * (CS / VI) + uu], &bn[uu * VI]);
+        load_vec<T, VI, NT>(pb + fo.in_u[cn * (CS / VI) + uu], &bn[uu * VI]);
       const T* A = As[c & 1];
 #pragma unroll
       for (int ss = 0; ss < CS; ++ss)
@@ -251,8 +272,13 @@ __global__ void __launch_bounds__(kBlock)
       __syncthreads();
     }
 
-    if (m < n_orbits) {
-      T* p = out + bout + lo;
+    const uint32_t mcur = m;
+    const int64_t boutcur = bout;
+    m = mn;
+    bout = boutn;
+    pin = pinn;
+    if (mcur < n_orbits) {
+      T* p = out + boutcur + lo;
 #pragma unroll
       for (int v = 0; v < 4 * MB / VO; ++v) {
         T vals[VO];

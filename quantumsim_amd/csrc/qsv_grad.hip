@@ -350,6 +350,99 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+
+// PTMs above 64 x 64 (qutrit pairs 81 x 81, ...): target dims and strides,
+// the entry offsets decoded by the kernel (mixed radix, last target fastest).
+constexpr int kGradBigMaxSide = 256;
+constexpr int kGradBigMaxEntries = 8192;  // dout x din
+constexpr int kGradBigOrbits = 8;         // orbits staged in LDS per step
+constexpr int kGradBigBlocks = 64;
+struct BigGeo {
+  int32_t k;
+  int32_t din_t[QSV_MAX_TARGETS], dout_t[QSV_MAX_TARGETS];
+  int64_t sin_t[QSV_MAX_TARGETS], sout_t[QSV_MAX_TARGETS];
+};
+
+__host__ __device__ inline int64_t big_offset(int k, const int32_t* d, const int64_t* st, int e) {
+  int64_t off = 0;
+  for (int j = k - 1; j >= 0; --j) {
+    off += int64_t(e % d[j]) * st[j];
+    e /= d[j];
+  }
+  return off;
+}
+
+// PTMs above 64 x 64 (qutrit pairs, ...): dP[o][i] = sum over orbits of
+// ybar[o] x[i] on the VALU in float64.  A block takes chunks of
+// kGradBigOrbits orbits (chunk c, c + gridDim.x, ...), stages their x and
+// ybar entries in LDS (offsets decoded from the targets' dims and strides),
+// and thread t accumulates entries (o, i) = t, t + kBlock, ... of the
+// block's partial tile (same layout as the other gradient kernels, summed
+// by k_ptm_grad_final).
+constexpr int kGradBigPerThread = kGradBigMaxEntries / kBlock;
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_ptm_grad_big(const T* __restrict__ x, const T* __restrict__ ybar, const Geom g,
+                   const BigGeo bg, const uint32_t n_orbits, double* __restrict__ partial) {
+  __shared__ int64_t so_in[kGradBigMaxSide], so_out[kGradBigMaxSide];
+  __shared__ double sx[kGradBigOrbits * kGradBigMaxSide], sy[kGradBigOrbits * kGradBigMaxSide];
+  const uint32_t b = blockIdx.y;
+  x += int64_t(b) * g.bs_in;
+  ybar += int64_t(b) * g.bs_out;
+  const int tid = threadIdx.x, din = g.din, dout = g.dout, dd = din * dout;
+  for (int e = tid; e < din; e += kBlock) so_in[e] = big_offset(bg.k, bg.din_t, bg.sin_t, e);
+  for (int e = tid; e < dout; e += kBlock) so_out[e] = big_offset(bg.k, bg.dout_t, bg.sout_t, e);
+  double acc[kGradBigPerThread];
+#pragma unroll
+  for (int r = 0; r < kGradBigPerThread; ++r) acc[r] = 0.0;
+  const uint32_t nchunks = (n_orbits + kGradBigOrbits - 1) / kGradBigOrbits;
+  __syncthreads();
+  for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    for (int f = tid; f < kGradBigOrbits * din; f += kBlock) {
+      const int ol = f / din, i = f - ol * din;
+      const uint32_t m = c * kGradBigOrbits + ol;
+      double v = 0.0;
+      if (m < n_orbits) {
+        int64_t bin, bout;
+        orbit_base(g, m, bin, bout);
+        v = double(x[bin + so_in[i]]);
+      }
+      sx[ol * kGradBigMaxSide + i] = v;
+    }
+    for (int f = tid; f < kGradBigOrbits * dout; f += kBlock) {
+      const int ol = f / dout, o = f - ol * dout;
+      const uint32_t m = c * kGradBigOrbits + ol;
+      double v = 0.0;
+      if (m < n_orbits) {
+        int64_t bin, bout;
+        orbit_base(g, m, bin, bout);
+        v = double(ybar[bout + so_out[o]]);
+      }
+      sy[ol * kGradBigMaxSide + o] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kGradBigPerThread; ++r) {
+      const int j = tid + r * kBlock;
+      if (j < dd) {
+        const int o = j / din, i = j - o * din;
+        double s = acc[r];
+#pragma unroll
+        for (int ol = 0; ol < kGradBigOrbits; ++ol)
+          s += sy[ol * kGradBigMaxSide + o] * sx[ol * kGradBigMaxSide + i];
+        acc[r] = s;
+      }
+    }
+    __syncthreads();
+  }
+  double* out = partial + (int64_t(b) * gridDim.x + blockIdx.x) * dd;
+#pragma unroll
+  for (int r = 0; r < kGradBigPerThread; ++r) {
+    const int j = tid + r * kBlock;
+    if (j < dd) out[j] = acc[r];
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     k_ptm_grad_final(const double* __restrict__ partial, int nblocks, int dd,
@@ -374,6 +467,8 @@ struct GradPlan {
   int vn;    // orbits per 16-byte lane vector (1: scalar loads)
   int quad;  // 64 x 64 with a unit-stride lowest index: 16-byte loads of 4 entries
   int64_t n_in, n_out;
+  int big;   // above 64 on a side: k_ptm_grad_big, offsets decoded on the device
+  BigGeo bg;
 };
 
 // Tile of the LDS-staged gradient (float64 states only).
@@ -419,12 +514,23 @@ static int make_grad_plan(int elem_bytes, int rank, const int64_t* dims, int k,
     din *= dims[targets[j]];
     dout *= dims_out_t[j];
   }
-  QSV_CHECK_ARG(din <= 64 && dout <= 64, "PTM gradient supports up to 64 x 64, got %lld x %lld",
-                (long long)dout, (long long)din);
+  QSV_CHECK_ARG(din <= kGradBigMaxSide && dout <= kGradBigMaxSide &&
+                    din * dout <= kGradBigMaxEntries,
+                "PTM gradient supports up to %d entries per side and %d entries, got %lld x %lld",
+                kGradBigMaxSide, kGradBigMaxEntries, (long long)dout, (long long)din);
   p.din = int(din);
   p.dout = int(dout);
   memset(&p.offs, 0, sizeof(p.offs));
-  for (int64_t i = 0; i < din; ++i) {
+  memset(&p.bg, 0, sizeof(p.bg));
+  p.big = din > 64 || dout > 64;
+  p.bg.k = k;
+  for (int j = 0; j < k; ++j) {
+    p.bg.din_t[j] = int32_t(dims[targets[j]]);
+    p.bg.dout_t[j] = int32_t(dims_out_t[j]);
+    p.bg.sin_t[j] = sin[targets[j]];
+    p.bg.sout_t[j] = sout[targets[j]];
+  }
+  for (int64_t i = 0; i < din && !p.big; ++i) {
     int64_t r = i, off = 0;
     for (int j = k - 1; j >= 0; --j) {
       off += (r % dims[targets[j]]) * sin[targets[j]];
@@ -432,7 +538,7 @@ static int make_grad_plan(int elem_bytes, int rank, const int64_t* dims, int k,
     }
     p.offs.off_in[i] = off;
   }
-  for (int64_t o = 0; o < dout; ++o) {
+  for (int64_t o = 0; o < dout && !p.big; ++o) {
     int64_t r = o, off = 0;
     for (int j = k - 1; j >= 0; --j) {
       off += (r % dims_out_t[j]) * sout[targets[j]];
@@ -477,7 +583,7 @@ static int make_grad_plan(int elem_bytes, int rank, const int64_t* dims, int k,
   // them aligned
   p.vn = 1;
   const int want = 16 / elem_bytes;
-  if (ng > 0 && g.gs_in[0] == 1 && g.gs_out[0] == 1 && g.gdiv[0].d % want == 0 &&
+  if (!p.big && ng > 0 && g.gs_in[0] == 1 && g.gs_out[0] == 1 && g.gdiv[0].d % want == 0 &&
       p.n_in % want == 0 && p.n_out % want == 0) {
     bool aligned = true;
     for (int j = 1; j < ng; ++j)
@@ -489,7 +595,8 @@ static int make_grad_plan(int elem_bytes, int rank, const int64_t* dims, int k,
   // 64 x 64 PTMs whose entries come in aligned runs of four (the innermost
   // axis is a target of Pauli dimension 4)
   p.quad = 0;
-  if (p.vn == 1 && p.din == 64 && p.dout == 64 && p.n_in % want == 0 && p.n_out % want == 0) {
+  if (!p.big && p.vn == 1 && p.din == 64 && p.dout == 64 && p.n_in % want == 0 &&
+      p.n_out % want == 0) {
     bool ok = true;
     for (int e = 0; e < 64; ++e)
       ok = ok && p.offs.off_in[e] == p.offs.off_in[e & ~3] + (e & 3) &&
@@ -524,7 +631,24 @@ static int grad64_blocks(uint32_t n_orbits, int batch) {
   return int(b < 1 ? 1 : (b > kGrad64Blocks ? kGrad64Blocks : b));
 }
 
-static int max_grad_blocks(const GradPlan& p) { return grad_wide(p) ? kGrad64Blocks : kGradBlocks; }
+// Above 64 x 64: at least 256 orbits per block, ~1024 blocks over the
+// batch, at most kGradBigBlocks per state (partial tiles up to 64 KB).
+static int grad_big_blocks(uint32_t n_orbits, int batch) {
+  uint32_t b = (n_orbits + 255) / 256;
+  const uint32_t fill = uint32_t((1024 + std::max(batch, 1) - 1) / std::max(batch, 1));
+  b = std::min(b, fill);
+  return int(b < 1 ? 1 : (b > kGradBigBlocks ? kGradBigBlocks : b));
+}
+
+static int max_grad_blocks(const GradPlan& p) {
+  return p.big ? kGradBigBlocks : (grad_wide(p) ? kGrad64Blocks : kGradBlocks);
+}
+
+static int default_grad_blocks(const GradPlan& p, int batch) {
+  return p.big ? grad_big_blocks(p.n_orbits, batch)
+               : (grad_wide(p) ? grad64_blocks(p.n_orbits, batch)
+                               : grad_blocks(p.n_orbits, batch));
+}
 
 // Fused backward of a dense 64 x 64 block (both element types) when the
 // innermost axis is a unit-stride bystander:
@@ -731,6 +855,11 @@ static void launch_grad(const GradPlan& p, int vn, bool quad, dim3 grid, const v
                         const void* ybar, double* partial, hipStream_t s) {
   const T* xx = static_cast<const T*>(x);
   const T* yy = static_cast<const T*>(ybar);
+  if (p.big) {
+    hipLaunchKernelGGL(k_ptm_grad_big<T>, grid, dim3(kBlock), 0, s, xx, yy, p.g, p.bg,
+                       p.n_orbits, partial);
+    return;
+  }
   if (grad_wide(p)) {
     if (grad64_staged(p, int(sizeof(T)), vn))
       hipLaunchKernelGGL((k_ptm_grad64s<T, 16 / sizeof(T), kGrad64sTile>), grid, dim3(kBlock), 0,
@@ -784,10 +913,7 @@ static int ptm_grad_run(int dtype, const void* x, const void* ybar, int batch, i
   QSV_CHECK_ARG(!tile || tile->grad_blocks <= max_grad_blocks(p),
                 "tile: grad_blocks %d above %d for a %d x %d PTM", tile ? tile->grad_blocks : 0,
                 max_grad_blocks(p), p.dout, p.din);
-  int nb = tile && tile->grad_blocks > 0
-               ? tile->grad_blocks
-               : (grad_wide(p) ? grad64_blocks(p.n_orbits, batch)
-                               : grad_blocks(p.n_orbits, batch));
+  int nb = tile && tile->grad_blocks > 0 ? tile->grad_blocks : default_grad_blocks(p, batch);
   const int64_t per_block = int64_t(batch) * p.din * p.dout * 8;
   if (shrink) {
     QSV_CHECK_ARG(per_block <= partial_bytes,
@@ -1110,6 +1236,71 @@ static int emulate_grad64s(const GradPlan& p, int es, const double* x, const dou
   return QSV_OK;
 }
 
+// Host replay of k_ptm_grad_big + k_ptm_grad_final for one state: the
+// decoded offsets, every staged load in bounds, every (entry, orbit) staged
+// exactly once, the LDS slots inside their rows.
+static int emulate_grad_big(const GradPlan& p, const double* x, const double* ybar, int nb,
+                            double* dptm) {
+  const int din = p.din, dout = p.dout, dd = din * dout;
+  std::vector<int64_t> oin(din), oout(dout);
+  for (int e = 0; e < din; ++e) oin[e] = big_offset(p.bg.k, p.bg.din_t, p.bg.sin_t, e);
+  for (int e = 0; e < dout; ++e) oout[e] = big_offset(p.bg.k, p.bg.dout_t, p.bg.sout_t, e);
+  const uint32_t nchunks = (p.n_orbits + kGradBigOrbits - 1) / kGradBigOrbits;
+  std::vector<int> used_in(size_t(din) * p.n_orbits, 0), used_out(size_t(dout) * p.n_orbits, 0);
+  std::vector<double> partial(size_t(nb) * dd, 0.0), sx(kGradBigOrbits * kGradBigMaxSide),
+      sy(kGradBigOrbits * kGradBigMaxSide);
+  QSV_CHECK_ARG(dd <= kGradBigPerThread * kBlock, "big gradient: too many entries");
+  for (int blk = 0; blk < nb; ++blk) {
+    std::vector<double> acc(dd, 0.0);
+    for (uint32_t c = blk; c < nchunks; c += nb) {
+      for (int f = 0; f < kGradBigOrbits * din; ++f) {
+        const int ol = f / din, i = f - ol * din;
+        const uint32_t m = c * kGradBigOrbits + ol;
+        double v = 0.0;
+        if (m < p.n_orbits) {
+          int64_t bin, bout;
+          orbit_base(p.g, m, bin, bout);
+          const int64_t a = bin + oin[i];
+          QSV_CHECK_ARG(a >= 0 && a < p.n_in, "big gradient: x load out of bounds");
+          v = x[a];
+          ++used_in[size_t(i) * p.n_orbits + m];
+        }
+        sx[ol * kGradBigMaxSide + i] = v;
+      }
+      for (int f = 0; f < kGradBigOrbits * dout; ++f) {
+        const int ol = f / dout, o = f - ol * dout;
+        const uint32_t m = c * kGradBigOrbits + ol;
+        double v = 0.0;
+        if (m < p.n_orbits) {
+          int64_t bin, bout;
+          orbit_base(p.g, m, bin, bout);
+          const int64_t a = bout + oout[o];
+          QSV_CHECK_ARG(a >= 0 && a < p.n_out, "big gradient: ybar load out of bounds");
+          v = ybar[a];
+          ++used_out[size_t(o) * p.n_orbits + m];
+        }
+        sy[ol * kGradBigMaxSide + o] = v;
+      }
+      for (int j = 0; j < dd; ++j) {
+        const int o = j / din, i = j - o * din;
+        for (int ol = 0; ol < kGradBigOrbits; ++ol)
+          acc[j] += sy[ol * kGradBigMaxSide + o] * sx[ol * kGradBigMaxSide + i];
+      }
+    }
+    for (int j = 0; j < dd; ++j) partial[size_t(blk) * dd + j] = acc[j];
+  }
+  for (size_t j = 0; j < used_in.size(); ++j)
+    QSV_CHECK_ARG(used_in[j] == 1, "big gradient: x entry staged %d times", used_in[j]);
+  for (size_t j = 0; j < used_out.size(); ++j)
+    QSV_CHECK_ARG(used_out[j] == 1, "big gradient: ybar entry staged %d times", used_out[j]);
+  for (int j = 0; j < dd; ++j) {
+    double sum = 0.0;
+    for (int b = 0; b < nb; ++b) sum += partial[size_t(b) * dd + j];
+    dptm[j] = sum;
+  }
+  return QSV_OK;
+}
+
 extern "C" int qsv_emulate_ptm_grad(int dtype, const double* x, const double* ybar, int rank,
                                     const int64_t* dims, int k, const int32_t* targets,
                                     const int32_t* dims_out_t, double* dptm,
@@ -1124,7 +1315,12 @@ extern "C" int qsv_emulate_ptm_grad(int dtype, const double* x, const double* yb
   QSV_CHECK_ARG(grad_blocks_forced >= 0 && grad_blocks_forced <= max_grad_blocks(p),
                 "grad_blocks outside [0, %d]", max_grad_blocks(p));
   const bool staged = grad64_staged(p, dtype_size(dtype), p.vn);
-  if (vn_out) *vn_out = p.vn | (p.quad << 8) | (staged ? 512 : 0);
+  if (vn_out) *vn_out = p.vn | (p.quad << 8) | (staged ? 512 : 0) | (p.big ? 1024 : 0);
+  if (p.big)
+    return emulate_grad_big(p, x, ybar,
+                            grad_blocks_forced > 0 ? grad_blocks_forced
+                                                   : grad_big_blocks(p.n_orbits, 1),
+                            dptm);
   if (staged)
     return emulate_grad64s(p, dtype_size(dtype), x, ybar,
                            grad_blocks_forced > 0 ? grad_blocks_forced

@@ -391,11 +391,12 @@ class PauliVectorHIP(PauliVectorBase):
     def diagonal(self, *, get_data=True):
         """Populations <j|rho|j>, flat, length prod(dim_hilbert), float64
         (quantumsim/pauli_vectors/numpy.py:66-80); levels missing from a
-        subbasis read 0.  ``get_data=False`` returns the device tensor."""
-        d = self._diagonal_dev()
+        subbasis read 0.  ``get_data=False`` returns the same flat vector
+        as a device tensor (no host copy)."""
+        d = self._diagonal_dev().reshape(-1)
         if not get_data:
             return d
-        return d.reshape(-1).cpu().numpy()
+        return d.cpu().numpy()
 
     def multitake(self, *indices, get_data=True):
         """``to_pv()[np.ix_(*indices)]``, one index list per qudit, gathered

@@ -34,6 +34,8 @@ def main(argv=None):
     ap.add_argument('--qubits', type=int, default=14)
     ap.add_argument('--dtype', default='float64')
     ap.add_argument('--reps', type=int, default=3)
+    ap.add_argument('--block-kernel', type=int, default=1, choices=[0, 1],
+                    help='batched: window blocks by the block kernel (1) or torch products (0)')
     a = ap.parse_args(argv)
     import torch
     import quantumsim_amd as qa
@@ -80,7 +82,8 @@ def main(argv=None):
         # the benchmark's memory budget: kept states may take 60 % of the
         # free device memory (every state is kept)
         cb = CircuitBatch(random_circuit_batch(512, seed=2024), dtype=torch.float32,
-                          checkpoint_budget=0.6 * torch.cuda.mem_get_info()[0])
+                          checkpoint_budget=0.6 * torch.cuda.mem_get_info()[0],
+                          block_kernel=bool(a.block_kernel))
         w = cb.random_weights(7)
         cb.capture(w, autotune=True)       # tuning + warm-up + capture
         torch.cuda.synchronize()

@@ -382,3 +382,60 @@ def test_fused_backward_not_applicable():
     P = np.zeros((4,) * 6)
     assert emulate_backward(nat.QSV_F32, x, x, P, (2, 3, 4)) is None
     assert emulate_backward(nat.QSV_F64, x, x, P, (0, 1, 4)) is None
+
+
+# ------------------------------------------- PTMs above 64 x 64 (qudit pairs)
+BIG_GEOMS = [([9, 9, 4], (0, 1), [9, 9]), ([4, 9, 9], (2, 1), [9, 9]),
+             ([9, 4, 9], (0, 2), [9, 4]), ([9, 4, 2, 4], (0, 1, 2), [9, 4, 2]),
+             ([4, 9, 9, 2], (1, 2), [4, 9])]
+
+
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets,dout', BIG_GEOMS)
+def test_emulated_batched_table_matches_oracle(dims, targets, dout, dtype, rng):
+    """The batched path for PTMs that are neither inline nor dense 64 x 64
+    (two qutrits: 81 x 81, qutrit + qubit pairs, rectangular outputs): the
+    table kernel with offsets decoded on the device, replayed (the decoded
+    offsets are checked against the planner's)."""
+    pv = rng.standard_normal(dims)
+    din = [dims[t] for t in targets]
+    ptm = rng.standard_normal(list(dout) + din)
+    ref = oracle_apply(pv, ptm, targets)
+    out, path = emulate_tiled(dtype, pv, ptm, targets, nat.Tile(0, 0, 0, 0))
+    assert path == 3
+    assert np.allclose(out, ref, rtol=1e-12, atol=1e-12), (dims, targets, dout)
+    with pytest.raises(ValueError, match='no launch shape'):
+        emulate_tiled(dtype, pv, ptm, targets, nat.Tile(2, 64, 0, 0))
+
+
+@pytest.mark.parametrize('dtype', [nat.QSV_F64, nat.QSV_F32])
+@pytest.mark.parametrize('dims,targets,dout', BIG_GEOMS)
+def test_emulated_big_grad_matches_oracle(dims, targets, dout, dtype, rng):
+    """The gradient kernel for PTMs above 64 entries on a side, replayed:
+    decoded offsets, loads in bounds, every (entry, orbit) staged once, any
+    number of blocks per state; against numpy's einsum."""
+    n = len(dims)
+    oshape = list(dims)
+    for t, d in zip(targets, dout):
+        oshape[t] = d
+    x = rng.standard_normal(dims)
+    ybar = rng.standard_normal(oshape)
+    xl = list(range(n))
+    yl = list(range(n))
+    outl = [n + j for j in range(len(targets))] + [t for t in targets]
+    for j, t in enumerate(targets):
+        yl[t] = n + j
+    ref = np.einsum(ybar, yl, x, xl, outl)
+    for blocks in (0, 1, 3, 32):
+        got, vn = emulate_grad(dtype, x, ybar, targets, dout, blocks)
+        assert vn & 1024, vn
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (dims, targets, dout, blocks)
+
+
+def test_big_grad_limits():
+    """Four qubits (256 x 256 = 65536 entries) exceed the gradient's 8192."""
+    x = np.zeros([4] * 5)
+    with pytest.raises(ValueError, match='8192'):
+        emulate_grad(nat.QSV_F64, x, x, (0, 1, 2, 3), [4, 4, 4, 4])
+    with pytest.raises(ValueError, match='grad_blocks'):
+        emulate_grad(nat.QSV_F64, np.zeros([9, 9, 2]), np.zeros([9, 9, 2]), (0, 1), [9, 9], 65)

@@ -346,6 +346,10 @@ def test_diagonal_is_bit_exact_gather_for_general(rng):
     v = qa.PauliVectorHIP(bs, pv.copy())
     o = PauliVectorNumpy(_oracle_bases(bs), pv.copy())
     assert np.array_equal(v.diagonal(), o.diagonal())
+    # the device form is the same flat vector, no host copy
+    d = v.diagonal(get_data=False)
+    assert d.is_cuda and d.dim() == 1
+    assert np.array_equal(d.cpu().numpy(), o.diagonal())
 
 
 def test_partial_trace_quirks(rng):

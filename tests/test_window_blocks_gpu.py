@@ -33,7 +33,8 @@ def test_kernel_blocks_and_gradients_match_torch(n, dtype):
         assert a.dtype == dtype and a.is_contiguous()
         assert (a.double() - r.double()).abs().max().item() < tol
     ups = [torch.randn(r.shape, dtype=torch.float64, device=DEV) for r in ref]
-    g_ref = torch.autograd.grad(sum((r * u).sum() for r, u in zip(ref, ups)), [ang, t1, t2])
+    g_ref = torch.autograd.grad(sum((r * u).sum() for r, u in zip(ref, ups)), [ang, t1, t2],
+                                retain_graph=True)
     g_got = torch.autograd.grad(sum((a.double() * u).sum() for a, u in zip(got, ups)),
                                 [ang, t1, t2])
     for a, r in zip(g_got, g_ref):
