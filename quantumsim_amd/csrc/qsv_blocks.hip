@@ -50,40 +50,51 @@ struct WBCheck {
   }
 };
 
-__host__ __device__ inline int wb_dim(int nq) { return nq == 3 ? 64 : (nq == 2 ? 16 : 4); }
+__host__ __device__ __forceinline__ int wb_dim(int nq) { return nq == 3 ? 64 : (nq == 2 ? 16 : 4); }
 
 // Row of the block index with the target slots' digits given and the other
 // slots' digits taken from `rest` (most significant remaining slot first).
-__host__ __device__ inline int wb_row(int nq, int nt, const int* slots, const int* digits,
+__host__ __device__ __forceinline__ int wb_row(int nq, int nt, int s0, int s1, int d0, int d1,
                                       int rest) {
   int row = 0;
   int rem = nq - nt;  // remaining slots, filled from rest
   for (int s = 0; s < nq; ++s) {
     int d;
-    bool tgt = false;
-    int ti = 0;
-    for (int t = 0; t < nt; ++t)
-      if (slots[t] == s) {
-        tgt = true;
-        ti = t;
-      }
-    if (tgt) {
-      d = digits[ti];
+    if (s == s0) {
+      d = d0;
+    } else if (nt == 2 && s == s1) {
+      d = d1;
     } else {
       --rem;
-      int p = 1;
-      for (int r = 0; r < rem; ++r) p *= 4;
-      d = (rest / p) % 4;
+      d = (rest >> (2 * rem)) & 3;
     }
     row = row * 4 + d;
   }
   return row;
 }
 
+// T = CZ (Ra (x) Rb) of bond factor f for (b, w), 16 x 16 into Tf.
+__host__ __device__ __forceinline__ void wb_bond_T(int tid, int nthr, const qsv_wblock& blk,
+                                                   int f, const WBGeom& g, int b, int w,
+                                                   double* Tf, WBCheck& chk) {
+  const int layer = 2 * w + blk.f_layer[f];
+  const int qa = blk.qubits[blk.f_slot0[f]], qb = blk.qubits[blk.f_slot1[f]];
+  chk(qa >= 0 && qa < g.n && qb >= 0 && qb < g.n, "bond factor qubit out of range");
+  const double* Ra = g.rot + ((int64_t(b) * g.layers + layer) * g.n + qa) * 16;
+  const double* Rb = g.rot + ((int64_t(b) * g.layers + layer) * g.n + qb) * 16;
+  for (int e = tid; e < 256; e += nthr) {
+    const int i = e >> 4, j = e & 15;
+    double s = 0.0;
+    for (int m = 0; m < 16; ++m)
+      s += g.cz[i * 16 + m] * Ra[(m >> 2) * 4 + (j >> 2)] * Rb[(m & 3) * 4 + (j & 3)];
+    Tf[e] = s;
+  }
+}
+
 // Factor f of position `blk` for (b, w): F[f] (16 x 16 for a bond, 4 x 4 in
 // the first 16 entries for a single) and the bond's T = CZ (Ra (x) Rb).
 // Phase A: singles complete, bonds' T; phase B: bonds' F = (Da (x) Db) T.
-__host__ __device__ inline void wb_factors_a(int tid, int nthr, const qsv_wblock& blk,
+__host__ __device__ __forceinline__ void wb_factors_a(int tid, int nthr, const qsv_wblock& blk,
                                              const WBGeom& g, int b, int w, double* F, double* T,
                                              WBCheck& chk) {
   const bool keep1 = g.keep[b * g.nw + w] != 0;
@@ -104,22 +115,12 @@ __host__ __device__ inline void wb_factors_a(int tid, int nthr, const qsv_wblock
         F[f * 256 + e] = ident ? (r == c ? 1.0 : 0.0) : s;
       }
     } else {
-      const int qa = blk.qubits[blk.f_slot0[f]], qb = blk.qubits[blk.f_slot1[f]];
-      chk(qa >= 0 && qa < g.n && qb >= 0 && qb < g.n, "bond factor qubit out of range");
-      const double* Ra = g.rot + ((int64_t(b) * g.layers + layer) * g.n + qa) * 16;
-      const double* Rb = g.rot + ((int64_t(b) * g.layers + layer) * g.n + qb) * 16;
-      for (int e = tid; e < 256; e += nthr) {
-        const int i = e >> 4, j = e & 15;
-        double s = 0.0;
-        for (int m = 0; m < 16; ++m)
-          s += g.cz[i * 16 + m] * Ra[(m >> 2) * 4 + (j >> 2)] * Rb[(m & 3) * 4 + (j & 3)];
-        T[f * 256 + e] = s;
-      }
+      wb_bond_T(tid, nthr, blk, f, g, b, w, T + f * 256, chk);
     }
   }
 }
 
-__host__ __device__ inline void wb_factors_b(int tid, int nthr, const qsv_wblock& blk,
+__host__ __device__ __forceinline__ void wb_factors_b(int tid, int nthr, const qsv_wblock& blk,
                                              const WBGeom& g, int b, int w, double* F,
                                              const double* T) {
   const bool keep1 = g.keep[b * g.nw + w] != 0;
@@ -139,16 +140,21 @@ __host__ __device__ inline void wb_factors_b(int tid, int nthr, const qsv_wblock
   }
 }
 
-// M (D x D, row-major) <- E(F_f) M, or E(F_f)^T M with `transpose`: the
+// Row stride of the backward's D x D matrices in LDS: one double of padding
+// per row, so that lanes reading the same column of different rows (the
+// column apply, the partial trace) fall into different banks.
+__host__ __device__ __forceinline__ int wb_ld(int D) { return D + 1; }
+
+// M (D x D, row stride ld) <- E(F_f) M, or E(F_f)^T M with `transpose`: the
 // factor (DT = 4: one slot, 16: two slots) acts on the row index's target
-// digits.  Thread per (rest, column).
+// digits.  Thread per (rest, column): lanes walk the columns of a row.
 template <int DT>
-__host__ __device__ inline void wb_apply_t(int tid, int nthr, const qsv_wblock& blk, int f,
-                                           const double* F, double* M, bool transpose,
+__host__ __device__ __forceinline__ void wb_apply_t(int tid, int nthr, const qsv_wblock& blk, int f,
+                                           const double* F, double* M, int ld, bool transpose,
                                            WBCheck& chk) {
   constexpr int NT = DT == 16 ? 2 : 1;
   const int nq = blk.nq, D = wb_dim(nq);
-  const int slots[2] = {blk.f_slot0[f], blk.f_slot1[f]};
+  const int s0 = blk.f_slot0[f], s1 = blk.f_slot1[f];
   const int nrest = D / DT;
   const double* Ff = F + f * 256;
   for (int gi = tid; gi < nrest * D; gi += nthr) {
@@ -157,51 +163,98 @@ __host__ __device__ inline void wb_apply_t(int tid, int nthr, const qsv_wblock& 
     int rows[DT];
 #pragma unroll
     for (int t = 0; t < DT; ++t) {
-      const int dg[2] = {NT == 2 ? t >> 2 : t, t & 3};
-      rows[t] = wb_row(nq, NT, slots, dg, rest);
+      rows[t] = wb_row(nq, NT, s0, s1, NT == 2 ? t >> 2 : t, t & 3, rest);
       chk(rows[t] >= 0 && rows[t] < D, "row out of range");
-      v[t] = M[rows[t] * D + col];
+      v[t] = M[rows[t] * ld + col];
     }
 #pragma unroll
     for (int t = 0; t < DT; ++t) {
       double s = 0.0;
 #pragma unroll
       for (int u = 0; u < DT; ++u) s += (transpose ? Ff[u * DT + t] : Ff[t * DT + u]) * v[u];
-      M[rows[t] * D + col] = s;
+      M[rows[t] * ld + col] = s;
     }
   }
 }
 
-__host__ __device__ inline void wb_apply(int tid, int nthr, const qsv_wblock& blk, int f,
-                                         const double* F, double* M, bool transpose,
+__host__ __device__ __forceinline__ void wb_apply(int tid, int nthr, const qsv_wblock& blk, int f,
+                                         const double* F, double* M, int ld, bool transpose,
                                          WBCheck& chk) {
   if (blk.f_kind[f] == 2)
-    wb_apply_t<16>(tid, nthr, blk, f, F, M, transpose, chk);
+    wb_apply_t<16>(tid, nthr, blk, f, F, M, ld, transpose, chk);
   else
-    wb_apply_t<4>(tid, nthr, blk, f, F, M, transpose, chk);
+    wb_apply_t<4>(tid, nthr, blk, f, F, M, ld, transpose, chk);
 }
 
-__host__ __device__ inline void wb_identity(int tid, int nthr, int D, double* M) {
+// W (D x D, row stride ld) <- W E(F_f)^T: the factor acts on the column
+// index's target digits, W'[r][col(t, c)] = sum_u F[t][u] W[r][col(u, c)].
+// Thread per (row, rest): lanes walk the rows of a column (padded stride).
+template <int DT>
+__host__ __device__ __forceinline__ void wb_apply_col_t(int tid, int nthr, const qsv_wblock& blk, int f,
+                                               const double* F, double* W, int ld,
+                                               WBCheck& chk) {
+  constexpr int NT = DT == 16 ? 2 : 1;
+  const int nq = blk.nq, D = wb_dim(nq);
+  const int s0 = blk.f_slot0[f], s1 = blk.f_slot1[f];
+  const int nrest = D / DT;
+  const double* Ff = F + f * 256;
+  for (int gi = tid; gi < nrest * D; gi += nthr) {
+    const int r = gi % D, rest = gi / D;
+    double v[DT];
+    int cols[DT];
+#pragma unroll
+    for (int t = 0; t < DT; ++t) {
+      cols[t] = wb_row(nq, NT, s0, s1, NT == 2 ? t >> 2 : t, t & 3, rest);
+      chk(cols[t] >= 0 && cols[t] < D, "column out of range");
+      v[t] = W[r * ld + cols[t]];
+    }
+#pragma unroll
+    for (int t = 0; t < DT; ++t) {
+      double s = 0.0;
+#pragma unroll
+      for (int u = 0; u < DT; ++u) s += Ff[t * DT + u] * v[u];
+      W[r * ld + cols[t]] = s;
+    }
+  }
+}
+
+__host__ __device__ __forceinline__ void wb_apply_col(int tid, int nthr, const qsv_wblock& blk, int f,
+                                             const double* F, double* W, int ld, WBCheck& chk) {
+  if (blk.f_kind[f] == 2)
+    wb_apply_col_t<16>(tid, nthr, blk, f, F, W, ld, chk);
+  else
+    wb_apply_col_t<4>(tid, nthr, blk, f, F, W, ld, chk);
+}
+
+__host__ __device__ __forceinline__ void wb_identity(int tid, int nthr, int D, double* M) {
   for (int e = tid; e < D * D; e += nthr) M[e] = (e / D == e % D) ? 1.0 : 0.0;
 }
 
-// dF[ot][it] = sum over the other digits c and columns x of
-//              G[row(ot, c)][x] P[row(it, c)][x]
-__host__ __device__ inline void wb_contract(int tid, int nthr, const qsv_wblock& blk, int f,
-                                            const double* G, const double* P, double* dF,
-                                            WBCheck& chk) {
+__host__ __device__ __forceinline__ void wb_copy(int tid, int nthr, int D, int ld, const double* src,
+                                        double* dst) {
+  for (int e = tid; e < D * D; e += nthr) {
+    const int r = e / D, c = e - r * D;
+    dst[r * ld + c] = src[r * ld + c];
+  }
+}
+
+// dF[ot][it] = sum over the other digits c of W[row(ot, c)][row(it, c)]:
+// the gradient of the embedded factor E(F_f) = F_f (x) 1 when
+// W = dL/dE(F_f) (row stride ld).
+__host__ __device__ __forceinline__ void wb_trace(int tid, int nthr, const qsv_wblock& blk, int f,
+                                         const double* W, int ld, double* dF, WBCheck& chk) {
   const int nq = blk.nq, D = wb_dim(nq);
   const int nt = blk.f_kind[f] == 2 ? 2 : 1, dt = nt == 2 ? 16 : 4;
-  const int slots[2] = {blk.f_slot0[f], blk.f_slot1[f]};
+  const int s0 = blk.f_slot0[f], s1 = blk.f_slot1[f];
   for (int e = tid; e < dt * dt; e += nthr) {
     const int ot = e / dt, it = e % dt;
-    const int dgo[2] = {nt == 2 ? ot >> 2 : ot, ot & 3};
-    const int dgi[2] = {nt == 2 ? it >> 2 : it, it & 3};
+    const int o0 = nt == 2 ? ot >> 2 : ot, o1 = ot & 3;
+    const int i0 = nt == 2 ? it >> 2 : it, i1 = it & 3;
     double s = 0.0;
     for (int c = 0; c < D / dt; ++c) {
-      const int ro = wb_row(nq, nt, slots, dgo, c), ri = wb_row(nq, nt, slots, dgi, c);
-      chk(ro >= 0 && ro < D && ri >= 0 && ri < D, "contraction row out of range");
-      for (int x = 0; x < D; ++x) s += G[ro * D + x] * P[ri * D + x];
+      const int ro = wb_row(nq, nt, s0, s1, o0, o1, c), ri = wb_row(nq, nt, s0, s1, i0, i1, c);
+      chk(ro >= 0 && ro < D && ri >= 0 && ri < D, "trace entry out of range");
+      s += W[ro * ld + ri];
     }
     dF[e] = s;
   }
@@ -210,7 +263,7 @@ __host__ __device__ inline void wb_contract(int tid, int nthr, const qsv_wblock&
 // dF of factor f -> dR (written) and the (window, layer) part of dD.
 // Bond: X = (Da (x) Db)^T dF, dK = CZ^T X, dL = dF T^T; then the four 4 x 4
 // partial contractions.  Masked layer-1 factors write zeros.
-__host__ __device__ inline void wb_factor_grads_a(int tid, int nthr, const qsv_wblock& blk,
+__host__ __device__ __forceinline__ void wb_factor_grads_a(int tid, int nthr, const qsv_wblock& blk,
                                                   int f, const WBGeom& g, int b, int w,
                                                   const double* dF, double* X) {
   if (blk.f_kind[f] != 2) return;
@@ -226,9 +279,9 @@ __host__ __device__ inline void wb_factor_grads_a(int tid, int nthr, const qsv_w
   }
 }
 
-__host__ __device__ inline void wb_factor_grads_b(int tid, int nthr, const qsv_wblock& blk,
+__host__ __device__ __forceinline__ void wb_factor_grads_b(int tid, int nthr, const qsv_wblock& blk,
                                                   int f, const WBGeom& g, const double* dF,
-                                                  const double* X, const double* T, double* dK,
+                                                  const double* X, const double* Tf, double* dK,
                                                   double* dL) {
   if (blk.f_kind[f] != 2) return;
   for (int e = tid; e < 256; e += nthr) {
@@ -236,14 +289,14 @@ __host__ __device__ inline void wb_factor_grads_b(int tid, int nthr, const qsv_w
     double sk = 0.0, sl = 0.0;
     for (int m = 0; m < 16; ++m) {
       sk += g.cz[m * 16 + i] * X[m * 16 + j];
-      sl += dF[i * 16 + m] * T[f * 256 + j * 16 + m];
+      sl += dF[i * 16 + m] * Tf[j * 16 + m];
     }
     dK[e] = sk;
     dL[e] = sl;
   }
 }
 
-__host__ __device__ inline void wb_factor_grads_c(int tid, int nthr, const qsv_wblock& blk,
+__host__ __device__ __forceinline__ void wb_factor_grads_c(int tid, int nthr, const qsv_wblock& blk,
                                                   int f, const WBGeom& g, int b, int w,
                                                   const double* dF, const double* dK,
                                                   const double* dL, double* drot,
@@ -297,7 +350,7 @@ __host__ __device__ inline void wb_factor_grads_c(int tid, int nthr, const qsv_w
 }
 
 template <typename T>
-__host__ __device__ inline void wb_store(int tid, int nthr, const qsv_wblock& blk,
+__host__ __device__ __forceinline__ void wb_store(int tid, int nthr, const qsv_wblock& blk,
                                          const WBGeom& g, int b, int w, const double* M, T* out,
                                          int64_t out_elems, WBCheck& chk) {
   const int D = wb_dim(blk.nq);
@@ -308,14 +361,15 @@ __host__ __device__ inline void wb_store(int tid, int nthr, const qsv_wblock& bl
 }
 
 template <typename T>
-__host__ __device__ inline void wb_load(int tid, int nthr, const qsv_wblock& blk,
+__host__ __device__ __forceinline__ void wb_load(int tid, int nthr, const qsv_wblock& blk,
                                         const WBGeom& g, int b, int w, const T* dout, double* G,
                                         int64_t out_elems, WBCheck& chk) {
   const int D = wb_dim(blk.nq);
   const T* src = dout + blk.out_off + (int64_t(w) * g.batch + b) * D * D;
   chk(blk.out_off >= 0 && blk.out_off + (int64_t(w) * g.batch + b + 1) * D * D <= out_elems,
       "block gradient out of range");
-  for (int e = tid; e < D * D; e += nthr) G[e] = double(src[e]);
+  const int ld = wb_ld(D);
+  for (int e = tid; e < D * D; e += nthr) G[(e / D) * ld + e % D] = double(src[e]);
 }
 
 // ------------------------------------------------------------ kernels
@@ -325,7 +379,7 @@ __global__ void __launch_bounds__(kWB)
                     int64_t out_elems) {
   __shared__ double F[4 * 256], Tm[4 * 256], M[64 * 64];
   const int b = blockIdx.x, w = blockIdx.y;
-  const qsv_wblock blk = table[blockIdx.z];
+  const qsv_wblock& blk = table[blockIdx.z];  // uniform: scalar loads, no private copy
   const int tid = threadIdx.x;
   WBCheck chk;
   wb_factors_a(tid, kWB, blk, g, b, w, F, Tm, chk);
@@ -334,41 +388,58 @@ __global__ void __launch_bounds__(kWB)
   wb_factors_b(tid, kWB, blk, g, b, w, F, Tm);
   __syncthreads();
   for (int f = 0; f < blk.nf; ++f) {
-    wb_apply(tid, kWB, blk, f, F, M, false, chk);
+    wb_apply(tid, kWB, blk, f, F, M, wb_dim(blk.nq), false, chk);
     __syncthreads();
   }
   wb_store<T>(tid, kWB, blk, g, b, w, M, out, out_elems, chk);
 }
 
+// Backward of one block B = E_{nf-1} ... E_0 (E_f = F_f embedded).  With
+// G_f = (E_{nf-1} ... E_{f+1})^T dL/dB (G updated in place, last factor
+// first) and P_f = E_{f-1} ... E_0, dL/dE_f = G_f P_f^T: W = G_f is copied
+// and multiplied from the right by E_0^T, ..., E_{f-1}^T (column applies),
+// and dL/dF_f is W's partial trace over the digits F_f does not touch.
+// LDS: the factors, G, W and dF (76.8 KB: two workgroups per CU).  W holds
+// the bonds' T = CZ (Ra (x) Rb) while the factors are built, and after each
+// partial trace the factor-gradient scratch: X, dK, dL and the factor's T
+// (recomputed), at offsets 0, 256, 512 and 768.
 template <typename T>
 __global__ void __launch_bounds__(kWB)
     k_window_blocks_bwd(const qsv_wblock* __restrict__ table, const WBGeom g,
                         const T* __restrict__ dout, int64_t out_elems, double* __restrict__ drot,
                         double* __restrict__ ddamp) {
-  __shared__ double F[4 * 256], Tm[4 * 256], G[64 * 64], P[64 * 64];
-  __shared__ double dF[256], X[256], dK[256], dL[256];
+  __shared__ double F[4 * 256], G[64 * 65], W[64 * 65], dF[256];
   const int b = blockIdx.x, w = blockIdx.y;
-  const qsv_wblock blk = table[blockIdx.z];
-  const int tid = threadIdx.x, D = wb_dim(blk.nq);
+  const qsv_wblock& blk = table[blockIdx.z];  // uniform: scalar loads, no private copy
+  const int tid = threadIdx.x, D = wb_dim(blk.nq), ld = wb_ld(D);
+  double* const X = W;
+  double* const dK = W + 256;
+  double* const dL = W + 512;
+  double* const Tf = W + 768;
   WBCheck chk;
-  wb_factors_a(tid, kWB, blk, g, b, w, F, Tm, chk);
+  wb_factors_a(tid, kWB, blk, g, b, w, F, W, chk);
   wb_load<T>(tid, kWB, blk, g, b, w, dout, G, out_elems, chk);
   __syncthreads();
-  wb_factors_b(tid, kWB, blk, g, b, w, F, Tm);
+  wb_factors_b(tid, kWB, blk, g, b, w, F, W);
   __syncthreads();
   for (int f = blk.nf - 1; f >= 0; --f) {
-    wb_identity(tid, kWB, D, P);
-    __syncthreads();
-    for (int h = 0; h < f; ++h) {
-      wb_apply(tid, kWB, blk, h, F, P, false, chk);
+    const double* Wf = G;
+    if (f > 0) {
+      wb_copy(tid, kWB, D, ld, G, W);
       __syncthreads();
+      for (int h = 0; h < f; ++h) {
+        wb_apply_col(tid, kWB, blk, h, F, W, ld, chk);
+        __syncthreads();
+      }
+      Wf = W;
     }
-    wb_contract(tid, kWB, blk, f, G, P, dF, chk);
+    wb_trace(tid, kWB, blk, f, Wf, ld, dF, chk);
     __syncthreads();
-    if (f > 0) wb_apply(tid, kWB, blk, f, F, G, true, chk);
+    if (f > 0) wb_apply(tid, kWB, blk, f, F, G, ld, true, chk);
     wb_factor_grads_a(tid, kWB, blk, f, g, b, w, dF, X);
+    if (blk.f_kind[f] == 2) wb_bond_T(tid, kWB, blk, f, g, b, w, Tf, chk);
     __syncthreads();
-    wb_factor_grads_b(tid, kWB, blk, f, g, dF, X, Tm, dK, dL);
+    wb_factor_grads_b(tid, kWB, blk, f, g, dF, X, Tf, dK, dL);
     __syncthreads();
     wb_factor_grads_c(tid, kWB, blk, f, g, b, w, dF, dK, dL, drot, ddamp, chk);
     __syncthreads();
@@ -488,8 +559,12 @@ extern "C" int qsv_emulate_window_blocks(int n_pos, const qsv_wblock* table, int
   rc = check_table(table, n_pos, n_qubits, out_elems, batch, n_windows);
   if (rc) return rc;
   QSV_CHECK_ARG(out, "null pointer argument");
-  std::vector<double> F(4 * 256), Tm(4 * 256), M(64 * 64), G(64 * 64), P(64 * 64);
-  std::vector<double> dF(256), X(256), dK(256), dL(256);
+  std::vector<double> F(4 * 256), Tm(4 * 256), M(64 * 64), G(64 * 65), W(64 * 65);
+  std::vector<double> dF(256);
+  double* const X = W.data();
+  double* const dK = W.data() + 256;
+  double* const dL = W.data() + 512;
+  double* const Tf = W.data() + 768;
   WBCheck chk;
   auto each = [&](auto fn) {
     for (int t = 0; t < kWB; ++t) fn(t);
@@ -503,26 +578,34 @@ extern "C" int qsv_emulate_window_blocks(int n_pos, const qsv_wblock* table, int
         each([&](int t) { wb_identity(t, kWB, D, M.data()); });
         each([&](int t) { wb_factors_b(t, kWB, blk, g, b, w, F.data(), Tm.data()); });
         for (int f = 0; f < blk.nf; ++f)
-          each([&](int t) { wb_apply(t, kWB, blk, f, F.data(), M.data(), false, chk); });
+          each([&](int t) { wb_apply(t, kWB, blk, f, F.data(), M.data(), D, false, chk); });
         each([&](int t) { wb_store<double>(t, kWB, blk, g, b, w, M.data(), out, out_elems, chk); });
         if (!dout) continue;
         QSV_CHECK_ARG(drot && ddamp_part, "null gradient buffers");
-        each([&](int t) { wb_load<double>(t, kWB, blk, g, b, w, dout, G.data(), out_elems, chk); });
+        // the backward kernel's phases: factors rebuilt with W holding T
+        each([&](int t) {
+          wb_factors_a(t, kWB, blk, g, b, w, F.data(), W.data(), chk);
+          wb_load<double>(t, kWB, blk, g, b, w, dout, G.data(), out_elems, chk);
+        });
+        each([&](int t) { wb_factors_b(t, kWB, blk, g, b, w, F.data(), W.data()); });
+        const int ld = wb_ld(D);
         for (int f = blk.nf - 1; f >= 0; --f) {
-          each([&](int t) { wb_identity(t, kWB, D, P.data()); });
-          for (int h = 0; h < f; ++h)
-            each([&](int t) { wb_apply(t, kWB, blk, h, F.data(), P.data(), false, chk); });
-          each([&](int t) { wb_contract(t, kWB, blk, f, G.data(), P.data(), dF.data(), chk); });
-          if (f > 0)
-            each([&](int t) { wb_apply(t, kWB, blk, f, F.data(), G.data(), true, chk); });
-          each([&](int t) { wb_factor_grads_a(t, kWB, blk, f, g, b, w, dF.data(), X.data()); });
+          const double* Wf = G.data();
+          if (f > 0) {
+            each([&](int t) { wb_copy(t, kWB, D, ld, G.data(), W.data()); });
+            for (int h = 0; h < f; ++h)
+              each([&](int t) { wb_apply_col(t, kWB, blk, h, F.data(), W.data(), ld, chk); });
+            Wf = W.data();
+          }
+          each([&](int t) { wb_trace(t, kWB, blk, f, Wf, ld, dF.data(), chk); });
           each([&](int t) {
-            wb_factor_grads_b(t, kWB, blk, f, g, dF.data(), X.data(), Tm.data(), dK.data(),
-                              dL.data());
+            if (f > 0) wb_apply(t, kWB, blk, f, F.data(), G.data(), ld, true, chk);
+            wb_factor_grads_a(t, kWB, blk, f, g, b, w, dF.data(), X);
+            if (blk.f_kind[f] == 2) wb_bond_T(t, kWB, blk, f, g, b, w, Tf, chk);
           });
+          each([&](int t) { wb_factor_grads_b(t, kWB, blk, f, g, dF.data(), X, Tf, dK, dL); });
           each([&](int t) {
-            wb_factor_grads_c(t, kWB, blk, f, g, b, w, dF.data(), dK.data(), dL.data(), drot,
-                              ddamp_part, chk);
+            wb_factor_grads_c(t, kWB, blk, f, g, b, w, dF.data(), dK, dL, drot, ddamp_part, chk);
           });
         }
         QSV_CHECK_ARG(!chk.what, "window blocks: %s", chk.what);
