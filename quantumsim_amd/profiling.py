@@ -11,11 +11,14 @@ plain copy, then runs the kernels under study.
   fused   : dense 3- and 4-qubit PTMs (MFMA kernel) on middle and inner qubits
   diagonal: the float64 population readout (2^n-entry gather) of the state
   batched : the 512-circuit batch tuned and captured, then a marker kernel
-            (k_scale of 4,099 elements) and graph replays of forward + adjoint
+            (k_scale of 4,099 elements) and graph replays of forward + adjoint;
+            --block-kernel 0 builds the window blocks with torch products
   differentiable: the 12-qubit, 40-layer, 96-parameter circuit over 64
             parameter sets (float32, 3-qubit blocks, checkpoints every 8
             applications): one warm-up step, the marker, then forward +
             adjoint steps
+Both step workloads print the algorithmic bytes of one step by kernel
+family (quantumsim_amd.ledger) before the marker.
 """
 import argparse
 import os
@@ -39,6 +42,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     import torch
     import quantumsim_amd as qa
+    from quantumsim_amd.ledger import Ledger
     from quantumsim_amd import bases
     from quantumsim_amd.workloads import noisy_layers
 
@@ -65,8 +69,10 @@ def main(argv=None):
         def step():
             p = tc.populations(*params, torch.float32, checkpoint_every=8, fuse_qudits=3)
             return torch.autograd.grad(tc.kl(target, p).sum(), params)
-        step()
+        with Ledger(timed=False) as led:
+            step()
         torch.cuda.synchronize()
+        _print_ledger(led)
         m = torch.ones(MARKER_ELEMS, dtype=torch.float64, device='cuda')
         nat.check(nat.lib().qsv_scale(nat.QSV_F64, m.data_ptr(), m.numel(), 1.0,
                                       torch.cuda.current_stream().cuda_stream), 'scale')
@@ -85,8 +91,10 @@ def main(argv=None):
                           checkpoint_budget=0.6 * torch.cuda.mem_get_info()[0],
                           block_kernel=bool(a.block_kernel))
         w = cb.random_weights(7)
-        cb.capture(w, autotune=True)       # tuning + warm-up + capture
+        led = Ledger(timed=False)
+        cb.capture(w, autotune=True, ledger=led)   # tuning + warm-up + capture
         torch.cuda.synchronize()
+        _print_ledger(led)
         # marker: one k_scale launch (not used by the batch); the measured
         # replays follow it
         from quantumsim_amd import _native as nat
@@ -125,6 +133,14 @@ def main(argv=None):
     torch.cuda.synchronize()
     print('done: %d launches per rep' % len(gates), flush=True)
     return 0
+
+
+def _print_ledger(led):
+    """Algorithmic bytes of one step by kernel family (the step the measured
+    launches repeat)."""
+    for fam, row in sorted(led.rows.items()):
+        print('ledger %-20s launches %6d  algorithmic bytes %d' % (fam, row[0], row[1]),
+              flush=True)
 
 
 def src_bytes(dt):
