@@ -351,6 +351,172 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 
+// Float32 64 x 64 PTM gradient in the orbit-vector mode with the operand
+// tiles moved HBM -> LDS by the direct loads (global_load_lds_dwordx4: lane
+// l's 16 bytes land at the wave's LDS base + 16 l), so several tiles are in
+// flight without holding registers.  A tile is 32 orbits x 64 entries of x
+// and of ybar (8 KB each); wave-load j (0..7) covers entries 8j..8j+7, lane
+// l = e + 8 c taking entry 8j + e, orbits 4c..4c+3 (one 16-byte vector of
+// the innermost, unit-stride bystander).  Wave-load blocks are 1152 bytes
+// apart (128 bytes of padding) so that an MFMA operand read -- 16 entries
+// x 4 orbits, spread over two blocks -- hits 64 distinct banks.  Two
+// stages, in two separate LDS arrays: while the MFMAs of one tile read one
+// array, the loads of the next tile fill the other.  Wave w owns output
+// rows 16w..16w+15 (4 accumulator tiles): no cross-wave sum; the float32
+// accumulators are added into float64 ones every kDirFlush tiles.
+constexpr int kDirTile = 32;            // orbits per tile
+constexpr int kDirBlk = 1152;           // bytes per wave-load block (1024 + pad)
+constexpr int kDirArr = 8 * kDirBlk;    // bytes per operand tile
+constexpr int kDirFlush = 8;            // tiles per float32 partial sum
+
+// Workgroup barrier ordering LDS accesses only: a full __syncthreads()
+// also waits for every outstanding global load, including the direct loads
+// of the next tile that are meant to stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ void dir_issue(const float* x, const float* ybar, const Geom& g,
+                                          uint32_t t, int w, int c, const int64_t (&oin)[2],
+                                          const int64_t (&oout)[2], char* lds) {
+  int64_t bin, bout;
+  orbit_base(g, t * kDirTile + 4 * c, bin, bout);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int j = 2 * w + h;
+    __builtin_amdgcn_global_load_lds(x + bin + oin[h], lds + j * kDirBlk, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(ybar + bout + oout[h], lds + kDirArr + j * kDirBlk, 16, 0,
+                                     0);
+  }
+}
+
+// LDS byte offset of a pointer into a __shared__ array (the low 32 bits of
+// its generic address: the shared aperture is 4 GB aligned).
+__device__ __forceinline__ uint32_t lds_off(const void* p) {
+  return uint32_t(reinterpret_cast<uintptr_t>(p));
+}
+
+// Two k-steps (ks, ks + 1) of one tile: the A value (ybar) and the four B
+// values (x, one per column tile) of each, read from LDS in one asm block
+// that waits for its own reads (the compiler sees no LDS read here, so it
+// does not wait for the next tile's direct loads before them).  The outputs
+// are early-clobber: every read is issued before the block's wait, so no
+// output may share a register with the two address inputs.
+#define QSV_DIR_READ2(KS)                                                          \
+  asm volatile(                                                                    \
+      "ds_read_b32 %0, %10 offset:%c12\n"                                          \
+      "ds_read_b32 %1, %11 offset:%c12\n"                                          \
+      "ds_read_b32 %2, %11 offset:%c13\n"                                          \
+      "ds_read_b32 %3, %11 offset:%c14\n"                                          \
+      "ds_read_b32 %4, %11 offset:%c15\n"                                          \
+      "ds_read_b32 %5, %10 offset:%c16\n"                                          \
+      "ds_read_b32 %6, %11 offset:%c16\n"                                          \
+      "ds_read_b32 %7, %11 offset:%c17\n"                                          \
+      "ds_read_b32 %8, %11 offset:%c18\n"                                          \
+      "ds_read_b32 %9, %11 offset:%c19\n"                                          \
+      "s_waitcnt lgkmcnt(0)\n"                                                     \
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]),          \
+        "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7]), "=&v"(v[8]), "=&v"(v[9])            \
+      : "v"(ya), "v"(xb), "i"(128 * (KS)), "i"(128 * (KS) + 2 * kDirBlk),         \
+        "i"(128 * (KS) + 4 * kDirBlk), "i"(128 * (KS) + 6 * kDirBlk),             \
+        "i"(128 * (KS + 1)), "i"(128 * (KS + 1) + 2 * kDirBlk),                   \
+        "i"(128 * (KS + 1) + 4 * kDirBlk), "i"(128 * (KS + 1) + 6 * kDirBlk)      \
+      : "memory")
+
+__device__ __forceinline__ void dir_compute(const char* lds, int w, int lane,
+                                            Mfma<float>::acc (&acc)[4]) {
+  const int m = lane & 15, kk = lane >> 4;
+  const uint32_t ya = lds_off(lds + kDirArr + (2 * w + (m >> 3)) * kDirBlk + 16 * (m & 7) + 4 * kk);
+  const uint32_t xb = lds_off(lds + (m >> 3) * kDirBlk + 16 * (m & 7) + 4 * kk);
+  float v[10];
+#define QSV_DIR_STEP2(KS)                                       \
+  QSV_DIR_READ2(KS);                                            \
+  _Pragma("unroll") for (int h = 0; h < 2; ++h)                 \
+  _Pragma("unroll") for (int cb = 0; cb < 4; ++cb)              \
+    acc[cb] = Mfma<float>::op(v[5 * h], v[5 * h + 1 + cb], acc[cb]);
+  QSV_DIR_STEP2(0)
+  QSV_DIR_STEP2(2)
+  QSV_DIR_STEP2(4)
+  QSV_DIR_STEP2(6)
+#undef QSV_DIR_STEP2
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_ptm_grad64d(const float* __restrict__ x, const float* __restrict__ ybar, const Geom g,
+                  const GradOffs offs, const uint32_t n_orbits, double* __restrict__ partial) {
+  __shared__ __align__(16) char st0[2 * kDirArr];
+  __shared__ __align__(16) char st1[2 * kDirArr];
+  const uint32_t b = blockIdx.y;
+  x += int64_t(b) * g.bs_in;
+  ybar += int64_t(b) * g.bs_out;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int e = lane & 7, c = lane >> 3;
+  int64_t oin[2], oout[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    oin[h] = offs.off_in[8 * (2 * w + h) + e];
+    oout[h] = offs.off_out[8 * (2 * w + h) + e];
+  }
+  typedef Mfma<float>::acc Acc;
+  Acc acc[4];
+  double acc64[4][4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    acc[cb] = Acc(0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[cb][r] = 0.0;
+  }
+  auto flush = [&]() {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[cb][r] += double(acc[cb][r]);
+      acc[cb] = Acc(0);
+    }
+  };
+  const uint32_t ntiles = n_orbits / kDirTile, stride = gridDim.x;
+  uint32_t t = blockIdx.x;
+  int n_acc = 0;
+  if (t < ntiles) dir_issue(x, ybar, g, t, w, c, oin, oout, st0);
+  // two tiles per trip: st0 holds tile t, st1 tile t + stride
+  while (t < ntiles) {
+    const uint32_t t1 = t + stride;
+    if (t1 < ntiles) {
+      dir_issue(x, ybar, g, t1, w, c, oin, oout, st1);
+      __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4): this tile's loads done
+    } else {
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    }
+    lds_barrier();
+    dir_compute(st0, w, lane, acc);
+    if (++n_acc == kDirFlush) { flush(); n_acc = 0; }
+    lds_barrier();
+    if (t1 >= ntiles) break;
+    const uint32_t t2 = t1 + stride;
+    if (t2 < ntiles) {
+      dir_issue(x, ybar, g, t2, w, c, oin, oout, st0);
+      __builtin_amdgcn_s_waitcnt(0x0F74);
+    } else {
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    lds_barrier();
+    dir_compute(st1, w, lane, acc);
+    if (++n_acc == kDirFlush) { flush(); n_acc = 0; }
+    lds_barrier();
+    t = t2;
+  }
+  flush();
+  // slot r of lane (col, k) in tile cb: row 4k + r of the wave's 16 rows
+  const int col = lane & 15, k = lane >> 4;
+  double* out = partial + (int64_t(b) * gridDim.x + blockIdx.x) * 64 * 64;
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[(16 * w + 4 * k + r) * 64 + 16 * cb + col] = acc64[cb][r];
+}
+
 // PTMs above 64 x 64 (qutrit pairs 81 x 81, ...): target dims and strides,
 // the entry offsets decoded by the kernel (mixed radix, last target fastest).
 constexpr int kGradBigMaxSide = 256;
@@ -475,6 +641,13 @@ struct GradPlan {
 constexpr int kGrad64sTile = 16;
 
 // The staged kernel applies: float64, square 64 x 64, orbit vectors.
+// Float32 64 x 64 with 16-byte orbit vectors and whole 32-orbit tiles: the
+// direct-to-LDS kernel k_ptm_grad64d.
+static bool grad64_direct(const GradPlan& p, int elem_bytes, int vn) {
+  return elem_bytes == 4 && !p.big && p.din == 64 && p.dout == 64 && vn == 4 &&
+         p.n_orbits % kDirTile == 0;
+}
+
 static bool grad64_staged(const GradPlan& p, int elem_bytes, int vn) {
   return elem_bytes == 8 && p.din == 64 && p.dout == 64 && vn == 16 / elem_bytes;
 }
@@ -861,6 +1034,13 @@ static void launch_grad(const GradPlan& p, int vn, bool quad, dim3 grid, const v
     return;
   }
   if (grad_wide(p)) {
+    if constexpr (sizeof(T) == 4) {
+      if (grad64_direct(p, 4, vn)) {
+        hipLaunchKernelGGL(k_ptm_grad64d, grid, dim3(kBlock), 0, s, xx, yy, p.g, p.offs,
+                           p.n_orbits, partial);
+        return;
+      }
+    }
     if (grad64_staged(p, int(sizeof(T)), vn))
       hipLaunchKernelGGL((k_ptm_grad64s<T, 16 / sizeof(T), kGrad64sTile>), grid, dim3(kBlock), 0,
                          s, xx, yy, p.g, p.offs, p.n_orbits, partial);
@@ -1149,6 +1329,119 @@ static int emulate_grad64(const GradPlan& p, int es, const double* x, const doub
 // hardware index order (A = ybar, B = x, k = the four orbits of a k-step),
 // the block reduction's accumulator-to-entry map and the fixed-order final
 // sum.  Every orbit must be consumed exactly once.
+// Host replay of k_ptm_grad64d + k_ptm_grad_final for one state: every
+// direct load (16 bytes = orbits m..m+3 of one entry, in bounds, aligned,
+// the four orbits consecutive in memory) and the LDS slot it lands in;
+// every MFMA operand read checked to find, in the stage of the current
+// tile, the (entry, orbit) the product needs; every entry of the partial
+// tiles written once.
+static int emulate_grad64d(const GradPlan& p, const double* x, const double* ybar, int nb,
+                           double* dptm) {
+  QSV_CHECK_ARG(grad64_direct(p, 4, p.vn), "grad64d: not a direct geometry");
+  const uint32_t ntiles = p.n_orbits / kDirTile;
+  const int nf = 2 * kDirArr / 4;  // floats per stage
+  // per stage float: value, entry, orbit, array (0 x, 1 ybar), tile
+  std::vector<double> val(2 * size_t(nf));
+  std::vector<int64_t> tag(2 * size_t(nf) * 4);
+  std::vector<int> loaded(size_t(64) * p.n_orbits * 2, 0);
+  std::vector<double> partial(size_t(nb) * 64 * 64, 0.0);
+  for (int blk = 0; blk < nb; ++blk) {
+    std::vector<double> D(4 * 4 * 16 * 16, 0.0);  // [w][cb][row][col]
+    int stage = 0;
+    for (uint32_t t = blk; t < ntiles; t += nb, stage ^= 1) {
+      double* sv = val.data() + size_t(stage) * nf;
+      int64_t* stg = tag.data() + size_t(stage) * nf * 4;
+      std::fill(stg, stg + size_t(nf) * 4, int64_t(-1));
+      for (int w = 0; w < kBlock / 64; ++w)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int e = lane & 7, c = lane >> 3;
+          const uint32_t m = t * kDirTile + 4 * c;
+          int64_t bin, bout;
+          orbit_base(p.g, m, bin, bout);
+          for (int q = 1; q < 4; ++q) {
+            int64_t bi2, bo2;
+            orbit_base(p.g, m + q, bi2, bo2);
+            QSV_CHECK_ARG(bi2 == bin + q && bo2 == bout + q,
+                          "grad64d: orbits of a vector not consecutive");
+          }
+          for (int h = 0; h < 2; ++h) {
+            const int j = 2 * w + h, ent = 8 * j + e;
+            for (int arr = 0; arr < 2; ++arr) {
+              const int64_t a = arr == 0 ? bin + p.offs.off_in[ent] : bout + p.offs.off_out[ent];
+              const int64_t n_tot = arr == 0 ? p.n_in : p.n_out;
+              QSV_CHECK_ARG(a >= 0 && a + 4 <= n_tot && a % 4 == 0,
+                            "grad64d: direct load out of bounds or misaligned");
+              const int byte = arr * kDirArr + j * kDirBlk + 16 * lane;
+              QSV_CHECK_ARG(byte >= 0 && byte + 16 <= 2 * kDirArr, "grad64d: LDS slot");
+              for (int q = 0; q < 4; ++q) {
+                const int f = byte / 4 + q;
+                QSV_CHECK_ARG(stg[f * 4] == -1, "grad64d: LDS float written twice");
+                stg[f * 4 + 0] = ent;
+                stg[f * 4 + 1] = m + q;
+                stg[f * 4 + 2] = arr;
+                stg[f * 4 + 3] = t;
+                sv[f] = (arr == 0 ? x : ybar)[a + q];
+                ++loaded[(size_t(m + q) * 64 + ent) * 2 + arr];
+              }
+            }
+          }
+        }
+      // MFMA operands: lane (m, kk) of wave w, k-step ks
+      auto fetch = [&](int byte, int ent, uint32_t orb, int arr, double& out) -> int {
+        QSV_CHECK_ARG(byte >= 0 && byte + 4 <= 2 * kDirArr && byte % 4 == 0,
+                      "grad64d: operand read outside the stage");
+        const int f = byte / 4;
+        QSV_CHECK_ARG(stg[f * 4] == ent && stg[f * 4 + 1] == orb && stg[f * 4 + 2] == arr &&
+                          stg[f * 4 + 3] == int64_t(t),
+                      "grad64d: operand read finds the wrong entry or orbit");
+        out = sv[f];
+        return QSV_OK;
+      };
+      for (int w = 0; w < kBlock / 64; ++w)
+        for (int ks = 0; ks < kDirTile / 4; ++ks)
+          for (int kk = 0; kk < 4; ++kk) {
+            const uint32_t orb = t * kDirTile + 4 * ks + kk;
+            double A[16], B[4][16];
+            for (int mm = 0; mm < 16; ++mm) {
+              const int ya = kDirArr + (2 * w + (mm >> 3)) * kDirBlk + 16 * (mm & 7) + 4 * kk;
+              const int xb = (mm >> 3) * kDirBlk + 16 * (mm & 7) + 4 * kk;
+              int rc = fetch(ya + 128 * ks, 16 * w + mm, orb, 1, A[mm]);
+              if (rc) return rc;
+              for (int cb = 0; cb < 4; ++cb) {
+                rc = fetch(xb + 2 * cb * kDirBlk + 128 * ks, 16 * cb + mm, orb, 0, B[cb][mm]);
+                if (rc) return rc;
+              }
+            }
+            for (int cb = 0; cb < 4; ++cb)
+              for (int r = 0; r < 16; ++r)
+                for (int cc = 0; cc < 16; ++cc)
+                  D[((w * 4 + cb) * 16 + r) * 16 + cc] += A[r] * B[cb][cc];
+          }
+    }
+    // accumulator slot r of lane l: row 4 (l / 16) + r, column l % 16
+    std::vector<int> written(64 * 64, 0);
+    for (int w = 0; w < kBlock / 64; ++w)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int cb = 0; cb < 4; ++cb)
+          for (int r = 0; r < 4; ++r) {
+            const int row = 4 * (lane >> 4) + r, col = lane & 15;
+            const int idx = (16 * w + row) * 64 + 16 * cb + col;
+            ++written[idx];
+            partial[size_t(blk) * 4096 + idx] = D[((w * 4 + cb) * 16 + row) * 16 + col];
+          }
+    for (int i = 0; i < 4096; ++i)
+      QSV_CHECK_ARG(written[i] == 1, "grad64d: partial entry written %d times", written[i]);
+  }
+  for (size_t i = 0; i < loaded.size(); ++i)
+    QSV_CHECK_ARG(loaded[i] == 1, "grad64d: operand loaded %d times", loaded[i]);
+  for (int j = 0; j < 4096; ++j) {
+    double sum = 0.0;
+    for (int b = 0; b < nb; ++b) sum += partial[size_t(b) * 4096 + j];
+    dptm[j] = sum;
+  }
+  return QSV_OK;
+}
+
 // Host replay of k_ptm_grad64s + k_ptm_grad_final for one state: the
 // loader's vectors (in bounds, aligned, the orbits of a vector adjacent,
 // every entry and orbit loaded exactly once, LDS indices inside the padded
@@ -1315,7 +1608,15 @@ extern "C" int qsv_emulate_ptm_grad(int dtype, const double* x, const double* yb
   QSV_CHECK_ARG(grad_blocks_forced >= 0 && grad_blocks_forced <= max_grad_blocks(p),
                 "grad_blocks outside [0, %d]", max_grad_blocks(p));
   const bool staged = grad64_staged(p, dtype_size(dtype), p.vn);
-  if (vn_out) *vn_out = p.vn | (p.quad << 8) | (staged ? 512 : 0) | (p.big ? 1024 : 0);
+  const bool direct = grad64_direct(p, dtype_size(dtype), p.vn);
+  if (vn_out)
+    *vn_out = p.vn | (p.quad << 8) | (staged ? 512 : 0) | (p.big ? 1024 : 0) |
+              (direct ? 2048 : 0);
+  if (direct)
+    return emulate_grad64d(p, x, ybar,
+                           grad_blocks_forced > 0 ? grad_blocks_forced
+                                                  : grad64_blocks(p.n_orbits, 1),
+                           dptm);
   if (p.big)
     return emulate_grad_big(p, x, ybar,
                             grad_blocks_forced > 0 ? grad_blocks_forced

@@ -16,6 +16,7 @@ import torch
 
 import quantumsim_amd as qa
 from quantumsim_amd import bases
+from quantumsim_amd import _native as nat
 from quantumsim_amd.autograd import apply, run_circuit, run_schedule, Gate, diagonal
 from quantumsim_amd.workloads import TrainableCircuit, random_circuit_batch
 from quantumsim_amd.batch import CircuitBatch, plain_forward_backward
@@ -333,6 +334,45 @@ def test_ptm_grad64_kernel_matches_einsum(targets, dtype, tol):
     err = np.max(np.abs(got.reshape(ref.shape) - ref)) / np.max(np.abs(ref))
     print(targets, dtype, 'rel err %.3e' % err)
     assert err < tol
+
+
+@pytest.mark.parametrize('n,targets,B', [(6, (1, 2, 3), 2), (8, (0, 1, 2), 3), (10, (5, 6, 7), 3),
+                                         (11, (2, 3, 4), 2), (7, (4, 1, 0), 5)])
+def test_ptm_grad64_direct_kernel_matches_einsum(n, targets, B):
+    """The float32 64 x 64 gradient with direct HBM -> LDS loads (innermost
+    axis a bystander, whole 32-orbit tiles), for the planner's block count
+    and forced ones (1 .. 64 blocks per state, some with no tile), against
+    numpy's einsum of the same float32 inputs."""
+    from quantumsim_amd.autograd import _ptm_grad_raw
+    rng = np.random.default_rng(n + B)
+    shape = (4,) * n
+    x = rng.standard_normal((B,) + shape).astype(np.float32).astype(np.float64)
+    yb = rng.standard_normal((B,) + shape).astype(np.float32).astype(np.float64)
+    xl = [n + 1] + list(range(n))
+    yl = [n + 1] + list(range(n))
+    for j, t in enumerate(targets):
+        yl[1 + t] = n + 2 + j
+    ref = np.einsum(yb, yl, x, xl, [n + 1] + [n + 2 + j for j in range(3)] + list(targets))
+    ref = ref.reshape(B, 64, 64)
+    L = nat.lib()
+    xt = torch.tensor(x.reshape(B, -1), dtype=torch.float32, device=DEV)
+    yt = torch.tensor(yb.reshape(B, -1), dtype=torch.float32, device=DEV)
+    print('grad64 direct', n, targets, B, flush=True)
+    got = _ptm_grad_raw(xt, yt, shape, targets, (4, 4, 4)).double().cpu().numpy()
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-5
+    ws = L.qsv_ptm_grad_workspace_bytes(B, n, nat.i64(shape), 3, nat.i32(targets),
+                                        nat.i32((4, 4, 4)))
+    buf = torch.empty(ws, dtype=torch.uint8, device=DEV)
+    for nb in (1, 3, 17, 64):
+        out = torch.empty((B, 64, 64), dtype=torch.float32, device=DEV)
+        rc = L.qsv_ptm_grad_tiled(nat.QSV_F32, xt.data_ptr(), yt.data_ptr(), B, n,
+                                  nat.i64(shape), 3, nat.i32(targets), nat.i32((4, 4, 4)),
+                                  out.data_ptr(), 0, buf.data_ptr(), ws,
+                                  nat.ctypes.byref(nat.Tile(0, 0, nb, 0)),
+                                  torch.cuda.current_stream().cuda_stream)
+        nat.check(rc, 'grad')
+        err = np.max(np.abs(out.double().cpu().numpy() - ref)) / np.max(np.abs(ref))
+        assert err < 1e-5, (nb, err)
 
 
 def test_apply_dense64_gradcheck():

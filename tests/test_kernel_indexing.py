@@ -439,3 +439,37 @@ def test_big_grad_limits():
         emulate_grad(nat.QSV_F64, x, x, (0, 1, 2, 3), [4, 4, 4, 4])
     with pytest.raises(ValueError, match='grad_blocks'):
         emulate_grad(nat.QSV_F64, np.zeros([9, 9, 2]), np.zeros([9, 9, 2]), (0, 1), [9, 9], 65)
+
+
+@pytest.mark.parametrize('dims,targets', [([4] * 6, (1, 2, 3)), ([4] * 6, (2, 3, 4)),
+                                          ([4] * 7, (0, 1, 2)), ([4] * 6, (0, 2, 4)),
+                                          ([4] * 6, (3, 1, 0)), ([4, 4, 4, 4, 2, 4], (0, 1, 2))])
+def test_emulated_grad64_direct_matches_oracle(dims, targets, rng):
+    """Float32 64 x 64 gradient with 16-byte orbit vectors (the innermost axis
+    a bystander) and whole 32-orbit tiles: the direct-to-LDS kernel, replayed
+    (every direct load in bounds, its four orbits consecutive, its LDS slot;
+    every MFMA operand read finding its (entry, orbit) in the current stage;
+    every partial entry written once), for several block counts."""
+    n = len(dims)
+    x = rng.standard_normal(dims)
+    ybar = rng.standard_normal(dims)
+    xl = list(range(n))
+    yl = list(range(n))
+    outl = [n + j for j in range(len(targets))] + [t for t in targets]
+    for j, t in enumerate(targets):
+        yl[t] = n + j
+    ref = np.einsum(ybar, yl, x, xl, outl)
+    for blocks in (0, 1, 3, 7, 64):
+        got, vn = emulate_grad(nat.QSV_F32, x, ybar, targets, [4, 4, 4], blocks)
+        assert vn & 2048 and vn & 0xFF == 4, vn
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
+    # float64 states keep the register/LDS-staged kernels
+    assert not emulate_grad(nat.QSV_F64, x, ybar, targets, [4, 4, 4])[1] & 2048
+
+
+def test_grad64_direct_needs_whole_tiles(rng):
+    """16 orbits (five qubits) or an innermost target: not the direct kernel."""
+    x = rng.standard_normal([4] * 5)
+    assert not emulate_grad(nat.QSV_F32, x, x, (1, 2, 3), [4, 4, 4])[1] & 2048
+    x = rng.standard_normal([4] * 6)
+    assert not emulate_grad(nat.QSV_F32, x, x, (3, 4, 5), [4, 4, 4])[1] & 2048
