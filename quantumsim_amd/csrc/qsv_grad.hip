@@ -19,6 +19,7 @@
 // (deterministic).
 #include "qsv_fused.h"
 #include "qsv_geom.h"
+#include "qsv_lds.h"
 #include "qsv_mfma.h"
 
 #include <string.h>
@@ -369,15 +370,6 @@ constexpr int kDirBlk = 1152;           // bytes per wave-load block (1024 + pad
 constexpr int kDirArr = 8 * kDirBlk;    // bytes per operand tile
 constexpr int kDirFlush = 8;            // tiles per float32 partial sum
 
-// Workgroup barrier ordering LDS accesses only: a full __syncthreads()
-// also waits for every outstanding global load, including the direct loads
-// of the next tile that are meant to stay in flight across it.
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
-
 __device__ __forceinline__ void dir_issue(const float* x, const float* ybar, const Geom& g,
                                           uint32_t t, int w, int c, const int64_t (&oin)[2],
                                           const int64_t (&oout)[2], char* lds) {
@@ -390,12 +382,6 @@ __device__ __forceinline__ void dir_issue(const float* x, const float* ybar, con
     __builtin_amdgcn_global_load_lds(ybar + bout + oout[h], lds + kDirArr + j * kDirBlk, 16, 0,
                                      0);
   }
-}
-
-// LDS byte offset of a pointer into a __shared__ array (the low 32 bits of
-// its generic address: the shared aperture is 4 GB aligned).
-__device__ __forceinline__ uint32_t lds_off(const void* p) {
-  return uint32_t(reinterpret_cast<uintptr_t>(p));
 }
 
 // Two k-steps (ks, ks + 1) of one tile: the A value (ybar) and the four B
@@ -485,9 +471,9 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t t1 = t + stride;
     if (t1 < ntiles) {
       dir_issue(x, ybar, g, t1, w, c, oin, oout, st1);
-      __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4): this tile's loads done
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));  // this tile's loads done
     } else {
-      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     }
     lds_barrier();
     dir_compute(st0, w, lane, acc);
@@ -497,9 +483,9 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t t2 = t1 + stride;
     if (t2 < ntiles) {
       dir_issue(x, ybar, g, t2, w, c, oin, oout, st0);
-      __builtin_amdgcn_s_waitcnt(0x0F74);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
     } else {
-      __builtin_amdgcn_s_waitcnt(0x0F70);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
     }
     lds_barrier();
     dir_compute(st1, w, lane, acc);
