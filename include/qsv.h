@@ -300,9 +300,12 @@ int64_t qsv_ptm_grad_noalloc_scratch_bytes(void);
  * pass: xbar = P^T ybar (per state, P = ptm_dev + b * ptm_batch_stride,
  * row-major in the state's element type) and dptm (+)= sum over orbits of
  * ybar x^T as qsv_ptm_grad computes it; reads x and ybar once and writes
- * xbar (three streams instead of four).  workspace: qsv_ptm_grad_workspace_bytes
- * of the same geometry.  QSV_ENOTSUP unless the innermost axis is a
- * unit-stride bystander and the states are 16-byte aligned (callers then use
+ * xbar (three streams instead of four).  xbar is bitwise the transposed
+ * apply's (the same k-steps of four entries in the same order).  workspace:
+ * qsv_ptm_grad_workspace_bytes of the same geometry.  QSV_ENOTSUP unless the
+ * innermost axis is a unit-stride bystander and the states are 16-byte
+ * aligned, and for float32 unless the orbits come in 4-orbit vectors and
+ * whole 32-orbit tiles (the direct-to-LDS kernel; callers then use
  * qsv_apply_ptm_batched with the transposed PTM and qsv_ptm_grad).  xbar
  * must not alias x or ybar.  No counterpart in the reference; the adjoint
  * of the differentiable and batched paths. */

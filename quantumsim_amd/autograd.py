@@ -118,37 +118,69 @@ def _ptm_grad_raw(x, ybar, shape, targets, dims_out, out=None):
     return out
 
 
-def _backward_raw(x, gy, ptm, shape, targets, need_ptm_grad, fused=False):
+# (dtype, shape, targets) -> whether qsv_apply_ptm_batched_backward takes
+# the geometry (learned from its first call; QSV_ENOTSUP otherwise)
+_FUSED_GEOMETRY = {}
+
+
+def _fused_backward_raw(x, gy, ptm, shape, targets, gx_out=None, gp_out=None):
+    """(dL/dx, dL/dP) of y = P x for square dense 64 x 64 PTMs in one pass
+    (qsv_apply_ptm_batched_backward: x and gy read once, dL/dx written), or
+    None where it does not apply (then the transposed apply and the PTM
+    gradient kernel do the work).  ``gx_out`` (contiguous [B, N]) and
+    ``gp_out`` (contiguous, at least B PTMs) receive the results when
+    given.  ptm: [B or 1, 64, 64], contiguous, x's type."""
+    key = (x.dtype, tuple(shape), tuple(targets))
+    if ptm.shape[-1] != 64 or ptm.shape[-2] != 64 or _FUSED_GEOMETRY.get(key) is False:
+        return None
+    B = x.shape[0]
+    gx = torch.empty_like(gy) if gx_out is None else gx_out
+    if any(t.data_ptr() % 16 for t in (x, gy, gx)):
+        return None
+    L = nat.lib()
+    d = [shape[q] for q in targets]
+    pstride = 0 if ptm.shape[0] == 1 else 64 * 64
+    ws = nat.check(L.qsv_ptm_grad_workspace_bytes(B, len(shape), nat.i64(shape), len(targets),
+                                                  nat.i32(targets), nat.i32(d)), 'backward')
+    buf = _Workspace.get(x.device, ws)
+    gp = torch.empty((B, 64, 64), dtype=x.dtype, device=x.device) if gp_out is None else gp_out
+    assert gp.shape[0] >= B and tuple(gp.shape[1:]) == (64, 64) and gp.is_contiguous()
+    led = _ledger.active()
+    ev = led.begin() if led else None
+    rc = L.qsv_apply_ptm_batched_backward(
+        nat.dtype_code(x.dtype), x.data_ptr(), gy.data_ptr(), gx.data_ptr(), B, len(shape),
+        nat.i64(shape), len(targets), nat.i32(targets), ptm.data_ptr(), pstride, gp.data_ptr(),
+        0, buf.data_ptr(), ws, _stream())
+    if rc == -3:                           # not applicable: the separate kernels
+        _FUSED_GEOMETRY[key] = False
+        return None
+    nat.check(rc, 'backward')
+    _FUSED_GEOMETRY[key] = True
+    if led:
+        led.end(ev, _ledger.BWD_DENSE64,
+                x.element_size() * (B * (x.shape[1] + gy.shape[1] + gx.shape[1] + 64 * 64)
+                                    + ptm.numel()))
+    return gx, gp
+
+
+def _backward_raw(x, gy, ptm, shape, targets, need_ptm_grad, fused=None):
     """(dL/dx, dL/dP or None) of y = P x on the states x [B, N] for the
     incoming gy [B, N]: the transposed apply and the PTM gradient kernel.
 
-    ``fused=True`` uses the one-pass backward of square dense 64 x 64 PTMs
-    instead (qsv_apply_ptm_batched_backward, where it applies).  It reads
-    one stream less but runs at two waves per SIMD; measured on MI355X it is
-    slower than the two separate kernels (5.0 vs 3.7 ms for 64
-    twelve-qubit states; 0.44 vs 0.41 s of GPU time for the 512-circuit
-    step), so it is not the default.
+    ``fused`` (default: float32 states) uses the one-pass backward of square
+    dense 64 x 64 PTMs instead where it applies (qsv_apply_ptm_batched_backward:
+    three state streams instead of four).  For float32 that is the direct-to-LDS
+    kernel, measured on MI355X 15-20 % faster than the two kernels on
+    64 twelve-qubit states; float64 keeps the two kernels by default (its
+    one-pass kernel runs at two waves per SIMD and is slower).
     """
     d = [shape[q] for q in targets]
-    if fused and need_ptm_grad and ptm.shape[-1] == 64 and ptm.shape[-2] == 64:
-        B = x.shape[0]
-        L = nat.lib()
-        p = ptm.to(dtype=x.dtype).contiguous()
-        pstride = 0 if p.shape[0] == 1 else 64 * 64
-        ws = nat.check(L.qsv_ptm_grad_workspace_bytes(B, len(shape), nat.i64(shape),
-                                                      len(targets), nat.i32(targets),
-                                                      nat.i32(d)), 'backward')
-        buf = _Workspace.get(x.device, ws)
-        gx = torch.empty_like(gy)
-        gp = torch.empty((B, 64, 64), dtype=x.dtype, device=x.device)
-        rc = L.qsv_apply_ptm_batched_backward(
-            nat.dtype_code(x.dtype), x.data_ptr(), gy.data_ptr(), gx.data_ptr(), B, len(shape),
-            nat.i64(shape), len(targets), nat.i32(targets), p.data_ptr(), pstride,
-            gp.data_ptr(), 0, buf.data_ptr(), ws, _stream())
-        if rc == 0:
-            return gx, gp
-        if rc != -3:                       # anything but "not applicable" is an error
-            nat.check(rc, 'backward')
+    if fused is None:
+        fused = x.dtype == torch.float32
+    if fused and need_ptm_grad:
+        r = _fused_backward_raw(x, gy, ptm.to(dtype=x.dtype).contiguous(), shape, targets)
+        if r is not None:
+            return r
     gp = _ptm_grad_raw(x, gy, shape, targets, d) if need_ptm_grad else None
     gx = _apply_raw(gy, ptm.transpose(1, 2), shape, targets, d)
     return gx, gp
@@ -315,8 +347,15 @@ class _Schedule(torch.autograd.Function):
         # every (position, step) PTM is used by exactly one gate: its rows
         # of the running prefix are written in place, the rest stay zero
         grads = [torch.zeros_like(t) if nd else None for t, nd in zip(steps, need)]
-        # transposed PTMs for the adjoint applies, one copy per position
-        steps_t = [t.transpose(-1, -2).contiguous() for t in steps]
+        # transposed PTMs for the adjoint applies, one copy per position that
+        # needs them (float32 64 x 64 blocks run the one-pass backward)
+        steps_t = {}
+
+        def transposed(pos):
+            if pos not in steps_t:
+                steps_t[pos] = steps[pos].transpose(-1, -2).contiguous()
+            return steps_t[pos]
+        fused = gf.dtype == torch.float32
         bufs = [torch.empty_like(gf), torch.empty_like(gf)]
         cur = 0
         base = n_ck
@@ -338,11 +377,17 @@ class _Schedule(torch.autograd.Function):
                 for j in reversed(range(lo, hi)):
                     pos, w = gates[j]
                     gy = bufs[cur][:active]
-                    if grads[pos] is not None:
-                        _ptm_grad_raw(xs[j - lo], gy, shape, targets[pos], dims[pos],
-                                      out=grads[pos][w])
-                    _apply_raw(gy, steps_t[pos][w], shape, targets[pos], dims[pos],
-                               out=bufs[1 - cur][:active])
+                    done = None
+                    if fused and grads[pos] is not None:
+                        done = _fused_backward_raw(xs[j - lo], gy, steps[pos][w], shape,
+                                                   targets[pos], gx_out=bufs[1 - cur][:active],
+                                                   gp_out=grads[pos][w])
+                    if done is None:
+                        if grads[pos] is not None:
+                            _ptm_grad_raw(xs[j - lo], gy, shape, targets[pos], dims[pos],
+                                          out=grads[pos][w])
+                        _apply_raw(gy, transposed(pos)[w], shape, targets[pos], dims[pos],
+                                   out=bufs[1 - cur][:active])
                     cur = 1 - cur
         gx = bufs[cur] if ctx.needs_input_grad[0] else None
         return (gx, None, *grads)

@@ -503,6 +503,164 @@ __global__ void __launch_bounds__(kBlock)
     for (int r = 0; r < 4; ++r) out[(16 * w + 4 * k + r) * 64 + 16 * cb + col] = acc64[cb][r];
 }
 
+// LDS byte offset of k-step s of the adjoint apply below: entries 4s..4s+3
+// (wave-load block s / 2, entries 4 (s % 2) + kk of it).
+constexpr int bwd_koff(int s) { return (s >> 1) * kDirBlk + 64 * (s & 1); }
+
+typedef float qsv_f2 __attribute__((ext_vector_type(2)));
+
+// Eight k-steps of the adjoint apply's A operand (ybar^T: orbits 2m, 2m + 1
+// of entry 4s + kk, one 8-byte read per step), read and waited for in one
+// asm block as in QSV_DIR_READ2 (early-clobber outputs).
+#define QSV_BWD_READ8(S0)                                                           \
+  asm volatile(                                                                     \
+      "ds_read_b64 %0, %8 offset:%c9\n"                                             \
+      "ds_read_b64 %1, %8 offset:%c10\n"                                            \
+      "ds_read_b64 %2, %8 offset:%c11\n"                                            \
+      "ds_read_b64 %3, %8 offset:%c12\n"                                            \
+      "ds_read_b64 %4, %8 offset:%c13\n"                                            \
+      "ds_read_b64 %5, %8 offset:%c14\n"                                            \
+      "ds_read_b64 %6, %8 offset:%c15\n"                                            \
+      "ds_read_b64 %7, %8 offset:%c16\n"                                            \
+      "s_waitcnt lgkmcnt(0)\n"                                                      \
+      : "=&v"(av[0]), "=&v"(av[1]), "=&v"(av[2]), "=&v"(av[3]), "=&v"(av[4]),       \
+        "=&v"(av[5]), "=&v"(av[6]), "=&v"(av[7])                                    \
+      : "v"(ya), "i"(bwd_koff(S0)), "i"(bwd_koff(S0 + 1)), "i"(bwd_koff(S0 + 2)),   \
+        "i"(bwd_koff(S0 + 3)), "i"(bwd_koff(S0 + 4)), "i"(bwd_koff(S0 + 5)),        \
+        "i"(bwd_koff(S0 + 6)), "i"(bwd_koff(S0 + 7))                                \
+      : "memory")
+
+// xbar = P^T ybar on the staged tile, wave w's 16 entries 16w..16w+15:
+// x̄^T (32 orbits x 16 entries) = ybar^T (32 x 64) P[:, 16w..16w+15] as two
+// row tiles, row m of row tile rt being orbit 2m + rt; lane (n, kk) holds
+// B = P[4s + kk][16w + n] for all 16 k-steps in registers (pb).  Slot r of
+// lane l in row tile rt is orbit 8 (l / 16) + 2r + rt of entry 16w + l % 16,
+// so a lane ends with 8 consecutive orbits of one entry: two 16-byte stores.
+__device__ __forceinline__ void bwd_apply(const char* lds, const float (&pb)[16], int lane,
+                                          const Geom& g, uint32_t t, int64_t xo,
+                                          float* __restrict__ xbar) {
+  const int m = lane & 15, kk = lane >> 4;
+  const uint32_t ya = lds_off(lds + kDirArr + 16 * kk + 128 * (m >> 1) + 8 * (m & 1));
+  typedef Mfma<float>::acc Acc;
+  Acc a0 = Acc(0), a1 = Acc(0);
+  qsv_f2 av[8];
+  QSV_BWD_READ8(0);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    a0 = Mfma<float>::op(av[s].x, pb[s], a0);
+    a1 = Mfma<float>::op(av[s].y, pb[s], a1);
+  }
+  QSV_BWD_READ8(8);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    a0 = Mfma<float>::op(av[s].x, pb[8 + s], a0);
+    a1 = Mfma<float>::op(av[s].y, pb[8 + s], a1);
+  }
+  const uint32_t m0 = t * kDirTile + 8 * uint32_t(lane >> 4);
+  int64_t bin0, bout0, bin1, bout1;
+  orbit_base(g, m0, bin0, bout0);
+  orbit_base(g, m0 + 4, bin1, bout1);
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  f4 v0, v1;
+  v0[0] = a0[0]; v0[1] = a1[0]; v0[2] = a0[1]; v0[3] = a1[1];
+  v1[0] = a0[2]; v1[1] = a1[2]; v1[2] = a0[3]; v1[3] = a1[3];
+  *reinterpret_cast<f4*>(xbar + bin0 + xo) = v0;
+  *reinterpret_cast<f4*>(xbar + bin1 + xo) = v1;
+}
+
+// Fused float32 backward of a dense 64 x 64 block on k_ptm_grad64d's
+// staged tiles: per tile of 32 orbits, dP += ybar x^T (dir_compute, as in
+// k_ptm_grad64d) and xbar = P^T ybar from the same ybar tile in LDS
+// (bwd_apply), stored from the accumulators: x and ybar are read once, xbar
+// written once (three streams where the transposed apply plus the gradient
+// move four).  The wait before each tile counts the next tile's four loads
+// only: vector-memory loads retire in order among themselves, and the
+// stores issued between them (the previous tile's xbar) can only lower
+// the count, so vmcnt(4) still means this tile's loads are done.
+__global__ void __launch_bounds__(kBlock)
+    k_bwd64d(const float* __restrict__ x, const float* __restrict__ ybar,
+             float* __restrict__ xbar, const Geom g, const uint32_t n_orbits,
+             const float* __restrict__ ptm, const int64_t pstride, const GradOffs offs,
+             double* __restrict__ partial) {
+  __shared__ __align__(16) char st0[2 * kDirArr];
+  __shared__ __align__(16) char st1[2 * kDirArr];
+  const uint32_t b = blockIdx.y;
+  x += int64_t(b) * g.bs_in;
+  xbar += int64_t(b) * g.bs_in;
+  ybar += int64_t(b) * g.bs_out;
+  ptm += int64_t(b) * pstride;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int e = lane & 7, c = lane >> 3;
+  int64_t oin[2], oout[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    oin[h] = offs.off_in[8 * (2 * w + h) + e];
+    oout[h] = offs.off_out[8 * (2 * w + h) + e];
+  }
+  const int n = lane & 15, kk = lane >> 4;
+  const int64_t xo = offs.off_in[16 * w + n];
+  float pb[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) pb[s] = ptm[(4 * s + kk) * 64 + 16 * w + n];
+  __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // the PTM, before any tile load
+  typedef Mfma<float>::acc Acc;
+  Acc acc[4];
+  double acc64[4][4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    acc[cb] = Acc(0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[cb][r] = 0.0;
+  }
+  auto flush = [&]() {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[cb][r] += double(acc[cb][r]);
+      acc[cb] = Acc(0);
+    }
+  };
+  const uint32_t ntiles = n_orbits / kDirTile, stride = gridDim.x;
+  uint32_t t = blockIdx.x;
+  int n_acc = 0;
+  if (t < ntiles) dir_issue(x, ybar, g, t, w, c, oin, oout, st0);
+  while (t < ntiles) {
+    const uint32_t t1 = t + stride;
+    if (t1 < ntiles) {
+      dir_issue(x, ybar, g, t1, w, c, oin, oout, st1);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    lds_barrier();
+    dir_compute(st0, w, lane, acc);
+    bwd_apply(st0, pb, lane, g, t, xo, xbar);
+    if (++n_acc == kDirFlush) { flush(); n_acc = 0; }
+    lds_barrier();
+    if (t1 >= ntiles) break;
+    const uint32_t t2 = t1 + stride;
+    if (t2 < ntiles) {
+      dir_issue(x, ybar, g, t2, w, c, oin, oout, st0);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    lds_barrier();
+    dir_compute(st1, w, lane, acc);
+    bwd_apply(st1, pb, lane, g, t1, xo, xbar);
+    if (++n_acc == kDirFlush) { flush(); n_acc = 0; }
+    lds_barrier();
+    t = t2;
+  }
+  flush();
+  const int col = lane & 15, k = lane >> 4;
+  double* out = partial + (int64_t(b) * gridDim.x + blockIdx.x) * 64 * 64;
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[(16 * w + 4 * k + r) * 64 + 16 * cb + col] = acc64[cb][r];
+}
+
 // PTMs above 64 x 64 (qutrit pairs 81 x 81, ...): target dims and strides,
 // the entry offsets decoded by the kernel (mixed radix, last target fastest).
 constexpr int kGradBigMaxSide = 256;
@@ -809,8 +967,8 @@ static int default_grad_blocks(const GradPlan& p, int batch) {
                                : grad_blocks(p.n_orbits, batch));
 }
 
-// Fused backward of a dense 64 x 64 block (both element types) when the
-// innermost axis is a unit-stride bystander:
+// Fused backward of a dense 64 x 64 block of float64 states (float32 runs
+// k_bwd64d) when the innermost axis is a unit-stride bystander:
 //   xbar = P^T ybar   and   dP += sum over orbits ybar x^T
 // in one pass, reading x and ybar once from HBM and writing xbar (three
 // streams instead of the four of a transposed apply plus a gradient).  A
@@ -1322,8 +1480,9 @@ static int emulate_grad64(const GradPlan& p, int es, const double* x, const doub
 // tile, the (entry, orbit) the product needs; every entry of the partial
 // tiles written once.
 static int emulate_grad64d(const GradPlan& p, const double* x, const double* ybar, int nb,
-                           double* dptm) {
+                           double* dptm, const double* ptm = nullptr, double* xbar = nullptr) {
   QSV_CHECK_ARG(grad64_direct(p, 4, p.vn), "grad64d: not a direct geometry");
+  std::vector<int> xwritten(xbar ? size_t(p.n_in) : 0, 0);
   const uint32_t ntiles = p.n_orbits / kDirTile;
   const int nf = 2 * kDirArr / 4;  // floats per stage
   // per stage float: value, entry, orbit, array (0 x, 1 ybar), tile
@@ -1403,6 +1562,50 @@ static int emulate_grad64d(const GradPlan& p, const double* x, const double* yba
                 for (int cc = 0; cc < 16; ++cc)
                   D[((w * 4 + cb) * 16 + r) * 16 + cc] += A[r] * B[cb][cc];
           }
+      if (!xbar) continue;
+      // k_bwd64d's adjoint apply: lane (m, kk) of wave w reads orbits 2m,
+      // 2m + 1 of entry 4s + kk (8 bytes); B = P[4s + kk][16w + n]
+      for (int w = 0; w < kBlock / 64; ++w) {
+        double X[2][16][16] = {};  // [row tile][row m][column n]
+        for (int sk = 0; sk < 16; ++sk)
+          for (int kk = 0; kk < 4; ++kk) {
+            const int ent = 4 * sk + kk;
+            double A[2][16];
+            for (int mm = 0; mm < 16; ++mm) {
+              const int byte =
+                  kDirArr + 16 * kk + 128 * (mm >> 1) + 8 * (mm & 1) + bwd_koff(sk);
+              QSV_CHECK_ARG(byte % 8 == 0, "bwd64d: 8-byte read misaligned");
+              for (int rt = 0; rt < 2; ++rt) {
+                const int rc = fetch(byte + 4 * rt, ent, t * kDirTile + 2 * mm + rt, 1,
+                                     A[rt][mm]);
+                if (rc) return rc;
+              }
+            }
+            for (int rt = 0; rt < 2; ++rt)
+              for (int mm = 0; mm < 16; ++mm)
+                for (int nn = 0; nn < 16; ++nn)
+                  X[rt][mm][nn] += A[rt][mm] * ptm[ent * 64 + 16 * w + nn];
+          }
+        // slot r of lane l in row tile rt: row 4 (l / 16) + r, column l % 16
+        for (int lane = 0; lane < 64; ++lane) {
+          const int q = lane >> 4, nn = lane & 15;
+          for (int hv = 0; hv < 2; ++hv) {
+            const uint32_t m0 = t * kDirTile + 8 * q + 4 * hv;
+            int64_t bin, bout;
+            orbit_base(p.g, m0, bin, bout);
+            const int64_t a = bin + p.offs.off_in[16 * w + nn];
+            QSV_CHECK_ARG(a >= 0 && a + 4 <= p.n_in && a % 4 == 0,
+                          "bwd64d: xbar store out of bounds or misaligned");
+            for (int i = 0; i < 4; ++i) {
+              const int r = 2 * hv + (i >> 1), rt = i & 1;
+              QSV_CHECK_ARG(2 * (4 * q + r) + rt == 8 * q + 4 * hv + i,
+                            "bwd64d: store slot holds another orbit");
+              xbar[a + i] = X[rt][4 * q + r][nn];
+              ++xwritten[a + i];
+            }
+          }
+        }
+      }
     }
     // accumulator slot r of lane l: row 4 (l / 16) + r, column l % 16
     std::vector<int> written(64 * 64, 0);
@@ -1420,6 +1623,9 @@ static int emulate_grad64d(const GradPlan& p, const double* x, const double* yba
   }
   for (size_t i = 0; i < loaded.size(); ++i)
     QSV_CHECK_ARG(loaded[i] == 1, "grad64d: operand loaded %d times", loaded[i]);
+  for (size_t i = 0; i < xwritten.size(); ++i)
+    QSV_CHECK_ARG(xwritten[i] == 1, "bwd64d: xbar entry %lld written %d times", (long long)i,
+                  xwritten[i]);
   for (int j = 0; j < 4096; ++j) {
     double sum = 0.0;
     for (int b = 0; b < nb; ++b) sum += partial[size_t(b) * 4096 + j];
@@ -1716,6 +1922,11 @@ static int bwd64_plan(int dtype, int rank, const int64_t* dims, int k, const int
               "innermost axis");
     return QSV_ENOTSUP;
   }
+  if (dtype == QSV_F32 && !grad64_direct(p, 4, p.vn)) {
+    set_error("float32 fused backward needs 4-orbit vectors and whole %d-orbit tiles",
+              kDirTile);
+    return QSV_ENOTSUP;
+  }
   return QSV_OK;
 }
 
@@ -1723,6 +1934,30 @@ static int bwd64_blocks(const GradPlan& p, int batch, uint32_t cap) {
   const uint32_t tiles = (p.n_orbits + 16 * p.vn - 1) / (16 * p.vn);
   uint32_t nb = (tiles + kBlock / 64 - 1) / (kBlock / 64);
   nb = std::min<uint32_t>(nb, std::max<uint32_t>(1, cap / std::max(batch, 1)));
+  nb = std::min<uint32_t>(nb, kGrad64Blocks);
+  return int(std::max<uint32_t>(nb, 1));
+}
+
+// k_bwd64d: one round of resident blocks over the batch (occupancy x CUs /
+// batch per state; measured on 8-12-qubit states: 5-12 % faster than the
+// gradient's 512 orbits per block), at least 4 tiles per block.
+static int bwd64d_blocks(const GradPlan& p, int batch) {
+  static const uint32_t cap = [] {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(k_bwd64d), kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+      per_cu = 1;
+    (void)hipGetLastError();
+    return uint32_t(per_cu) * uint32_t(cus);
+  }();
+  const uint32_t tiles = p.n_orbits / kDirTile;
+  uint32_t nb = cap / uint32_t(std::max(batch, 1));
+  nb = std::min(nb, tiles / 4);
   nb = std::min<uint32_t>(nb, kGrad64Blocks);
   return int(std::max<uint32_t>(nb, 1));
 }
@@ -1768,23 +2003,23 @@ extern "C" int qsv_apply_ptm_batched_backward(int dtype, const void* x, const vo
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   const FusedOffs fo = bwd64_offsets(p);
-  const uint32_t cap = dtype == QSV_F64 ? bwd64_cap<double, 2>() : bwd64_cap<float, 4>();
-  const int nb = bwd64_blocks(p, batch, cap);
+  const bool direct = dtype == QSV_F32;  // bwd64_plan admits only k_bwd64d's geometries
+  const int nb = direct ? bwd64d_blocks(p, batch) : bwd64_blocks(p, batch, bwd64_cap<double, 2>());
   const int64_t need = int64_t(batch) * nb * 64 * 64 * 8;
   QSV_CHECK_ARG(workspace && workspace_bytes >= need, "fused backward needs %lld workspace bytes",
                 (long long)need);
   double* partial = static_cast<double*>(workspace);
   const dim3 grid(nb, batch);
-  if (dtype == QSV_F64)
+  if (direct)
+    hipLaunchKernelGGL(k_bwd64d, grid, dim3(kBlock), 0, s, static_cast<const float*>(x),
+                       static_cast<const float*>(ybar), static_cast<float*>(xbar), p.g,
+                       p.n_orbits, static_cast<const float*>(ptm_dev), ptm_batch_stride, p.offs,
+                       partial);
+  else
     hipLaunchKernelGGL((k_bwd64<double, 2>), grid, dim3(kBlock), 0, s,
                        static_cast<const double*>(x), static_cast<const double*>(ybar),
                        static_cast<double*>(xbar), p.g, p.n_orbits,
                        static_cast<const double*>(ptm_dev), ptm_batch_stride, p.offs, fo, partial);
-  else
-    hipLaunchKernelGGL((k_bwd64<float, 4>), grid, dim3(kBlock), 0, s,
-                       static_cast<const float*>(x), static_cast<const float*>(ybar),
-                       static_cast<float*>(xbar), p.g, p.n_orbits,
-                       static_cast<const float*>(ptm_dev), ptm_batch_stride, p.offs, fo, partial);
   QSV_CHECK_HIP(hipGetLastError());
   const dim3 g2((64 * 64 + kBlock - 1) / kBlock, batch);
   const double acc_flag = accumulate ? 1.0 : 0.0;
@@ -1802,7 +2037,8 @@ extern "C" int qsv_apply_ptm_batched_backward(int dtype, const void* x, const vo
 // lane's loads and stores (bounds, vector alignment, each xbar entry written
 // once) and the MFMA contraction in hardware index order with the
 // transposed PTM's fragment table; part B: the wide-gradient replay with
-// the kernel's four k-steps per tile.
+// the kernel's four k-steps per tile.  Float32 geometries of k_bwd64d
+// replay that kernel instead (emulate_grad64d with its apply half).
 extern "C" int qsv_emulate_apply_ptm_batched_backward(int dtype, const double* x,
                                                       const double* ybar, double* xbar,
                                                       int rank, const int64_t* dims, int k,
@@ -1817,6 +2053,10 @@ extern "C" int qsv_emulate_apply_ptm_batched_backward(int dtype, const double* x
   QSV_CHECK_ARG(grad_blocks >= 0 && grad_blocks <= kGrad64Blocks, "grad_blocks outside [0, %d]",
                 kGrad64Blocks);
   const int es = dtype_size(dtype), VN = p.vn, MB = 4, KS = 16;
+  if (dtype == QSV_F32)  // bwd64_plan admits only k_bwd64d's geometries
+    return emulate_grad64d(p, x, ybar,
+                           grad_blocks > 0 ? grad_blocks : grad64_blocks(p.n_orbits, 1), dptm,
+                           ptm, xbar);
   const int nb = grad_blocks > 0 ? grad_blocks : bwd64_blocks(p, 1, 256);
   const FusedOffs fo = bwd64_offsets(p);
   // fragment table of P^T, as the kernel gathers it

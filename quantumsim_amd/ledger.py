@@ -8,6 +8,9 @@ algorithm has to move:
   PTMs read;
 * PTM gradient (``qsv_ptm_grad``): both operand states, 2 * N * s per
   state for a square PTM, plus the gradient written;
+* fused backward (``qsv_apply_ptm_batched_backward``): x and ybar read,
+  xbar written, 3 * N * s per state, plus the PTMs read and the gradient
+  written;
 * persistent circuit kernels: the trajectory (the state before every gate,
   written forward and read back by the adjoint), the PTM pool, the float64
   populations and their gradients; the states themselves live in LDS.
@@ -23,12 +26,13 @@ lookup per launch.  No counterpart in the reference.
 import torch
 
 __all__ = ['Ledger', 'active', 'APPLY_DENSE64', 'APPLY_SMALL', 'GRAD_DENSE64', 'GRAD_SMALL',
-           'PERSISTENT_FWD', 'PERSISTENT_ADJ', 'KERNELS']
+           'BWD_DENSE64', 'PERSISTENT_FWD', 'PERSISTENT_ADJ', 'KERNELS']
 
 APPLY_DENSE64 = 'apply_dense64'
 APPLY_SMALL = 'apply_small'
 GRAD_DENSE64 = 'grad_dense64'
 GRAD_SMALL = 'grad_small'
+BWD_DENSE64 = 'bwd_dense64'
 PERSISTENT_FWD = 'persistent_forward'
 PERSISTENT_ADJ = 'persistent_adjoint'
 
@@ -38,6 +42,7 @@ KERNELS = {
     APPLY_SMALL: 'k_apply_small / k_apply_lds (PTMs up to 16 x 16)',
     GRAD_DENSE64: 'k_ptm_grad64 + k_ptm_grad_final',
     GRAD_SMALL: 'k_ptm_grad + k_ptm_grad_final',
+    BWD_DENSE64: 'k_bwd64d + k_ptm_grad_final (fused backward of 64 x 64 blocks)',
     PERSISTENT_FWD: 'k_circuits_fwd',
     PERSISTENT_ADJ: 'k_circuits_adj',
 }

@@ -356,11 +356,16 @@ def emulate_backward(dtype, x, ybar, ptm, targets, grad_blocks=0):
                                           ([4] * 6, (4, 1, 2))])
 def test_emulated_fused_backward_matches_oracle(dims, targets, dtype, rng):
     """The fused backward of a dense 64 x 64 block (xbar = P^T ybar and dP
-    in one pass), replayed: both parts against numpy."""
+    in one pass), replayed: both parts against numpy.  Float32 takes only
+    the direct kernel's geometries (whole 32-orbit tiles): QSV_ENOTSUP on
+    the others."""
     n = len(dims)
     x = rng.standard_normal(dims)
     ybar = rng.standard_normal(dims)
     P = rng.standard_normal((4,) * 6)
+    if dtype == nat.QSV_F32 and not emulate_grad(dtype, x, ybar, targets, [4, 4, 4])[1] & 2048:
+        assert emulate_backward(dtype, x, ybar, P, targets) is None
+        return
     ref_xbar = oracle_apply(ybar, np.ascontiguousarray(
         P.reshape(64, 64).T).reshape((4,) * 6), targets)
     xl, yl = list(range(n)), list(range(n))
@@ -372,6 +377,34 @@ def test_emulated_fused_backward_matches_oracle(dims, targets, dtype, rng):
         assert r is not None
         xbar, dp = r
         assert np.allclose(xbar, ref_xbar, rtol=1e-12, atol=1e-12), (dims, targets)
+        assert np.allclose(dp, ref_dp, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
+
+
+@pytest.mark.parametrize('dims,targets', [([4] * 6, (1, 2, 3)), ([4] * 6, (2, 3, 4)),
+                                          ([4] * 7, (0, 1, 2)), ([4] * 6, (0, 2, 4)),
+                                          ([4] * 6, (3, 1, 0)), ([4, 4, 4, 4, 2, 4], (0, 1, 2))])
+def test_emulated_fused_backward_direct_matches_oracle(dims, targets, rng):
+    """Float32 fused backward on whole 32-orbit tiles with 16-byte orbit
+    vectors (the geometries of the direct-to-LDS gradient, which the fused
+    backward then runs as k_bwd64d), replayed: the gradient half as for
+    k_ptm_grad64d, the adjoint apply's 8-byte operand reads (entry and
+    orbits checked in the current stage), every xbar store in bounds and
+    each xbar entry written once; both results against numpy for several
+    block counts."""
+    n = len(dims)
+    x = rng.standard_normal(dims)
+    ybar = rng.standard_normal(dims)
+    P = rng.standard_normal((4,) * 6)
+    assert emulate_grad(nat.QSV_F32, x, ybar, targets, [4, 4, 4])[1] & 2048
+    ref_xbar = oracle_apply(ybar, np.ascontiguousarray(
+        P.reshape(64, 64).T).reshape((4,) * 6), targets)
+    xl, yl = list(range(n)), list(range(n))
+    for j, t in enumerate(targets):
+        yl[t] = n + j
+    ref_dp = np.einsum(ybar, yl, x, xl, [n + j for j in range(3)] + list(targets)).reshape(64, 64)
+    for blocks in (0, 1, 3, 64):
+        xbar, dp = emulate_backward(nat.QSV_F32, x, ybar, P, targets, blocks)
+        assert np.allclose(xbar, ref_xbar, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
         assert np.allclose(dp, ref_dp, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
 
 
