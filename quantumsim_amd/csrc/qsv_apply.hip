@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(kBlock)
 // ------------------------------------------------------------ host side
 // Combined in+out bytes above which global accesses use the nontemporal
 // hint (QSV_NT_BYTES overrides it).
-static int64_t nt_threshold_bytes() {
+int64_t nt_threshold_bytes() {
   static int64_t v = [] {
     const char* e = getenv("QSV_NT_BYTES");
     return e ? int64_t(atoll(e)) : (int64_t(256) << 20);

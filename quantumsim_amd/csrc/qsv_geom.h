@@ -30,6 +30,10 @@ struct VT<T, 1> {
 // Global accesses of the streaming kernels.  NT = nontemporal (streaming)
 // hint, used when the vector is too large to stay cache-resident between
 // consecutive gates.
+// Bytes a launch moves (reads + writes over the batch) above which the
+// streaming kernels use nontemporal accesses (256 MiB; QSV_NT_BYTES).
+int64_t nt_threshold_bytes();
+
 template <bool NT, typename V>
 __device__ __forceinline__ V gload(const V* p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);

@@ -370,17 +370,21 @@ constexpr int kDirBlk = 1152;           // bytes per wave-load block (1024 + pad
 constexpr int kDirArr = 8 * kDirBlk;    // bytes per operand tile
 constexpr int kDirFlush = 8;            // tiles per float32 partial sum
 
+// Cache policy of the direct loads: bit 1 = nontemporal (streaming states
+// that do not stay in the caches until their next use).
+template <bool NT>
 __device__ __forceinline__ void dir_issue(const float* x, const float* ybar, const Geom& g,
                                           uint32_t t, int w, int c, const int64_t (&oin)[2],
                                           const int64_t (&oout)[2], char* lds) {
+  constexpr int kPol = NT ? 2 : 0;
   int64_t bin, bout;
   orbit_base(g, t * kDirTile + 4 * c, bin, bout);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int j = 2 * w + h;
-    __builtin_amdgcn_global_load_lds(x + bin + oin[h], lds + j * kDirBlk, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(x + bin + oin[h], lds + j * kDirBlk, 16, 0, kPol);
     __builtin_amdgcn_global_load_lds(ybar + bout + oout[h], lds + kDirArr + j * kDirBlk, 16, 0,
-                                     0);
+                                     kPol);
   }
 }
 
@@ -429,6 +433,7 @@ __device__ __forceinline__ void dir_compute(const char* lds, int w, int lane,
 #undef QSV_DIR_STEP2
 }
 
+template <bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_ptm_grad64d(const float* __restrict__ x, const float* __restrict__ ybar, const Geom g,
                   const GradOffs offs, const uint32_t n_orbits, double* __restrict__ partial) {
@@ -465,12 +470,12 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t ntiles = n_orbits / kDirTile, stride = gridDim.x;
   uint32_t t = blockIdx.x;
   int n_acc = 0;
-  if (t < ntiles) dir_issue(x, ybar, g, t, w, c, oin, oout, st0);
+  if (t < ntiles) dir_issue<NT>(x, ybar, g, t, w, c, oin, oout, st0);
   // two tiles per trip: st0 holds tile t, st1 tile t + stride
   while (t < ntiles) {
     const uint32_t t1 = t + stride;
     if (t1 < ntiles) {
-      dir_issue(x, ybar, g, t1, w, c, oin, oout, st1);
+      dir_issue<NT>(x, ybar, g, t1, w, c, oin, oout, st1);
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));  // this tile's loads done
     } else {
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
@@ -482,7 +487,7 @@ __global__ void __launch_bounds__(kBlock)
     if (t1 >= ntiles) break;
     const uint32_t t2 = t1 + stride;
     if (t2 < ntiles) {
-      dir_issue(x, ybar, g, t2, w, c, oin, oout, st0);
+      dir_issue<NT>(x, ybar, g, t2, w, c, oin, oout, st0);
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
     } else {
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
@@ -577,6 +582,7 @@ __device__ __forceinline__ void bwd_apply(const char* lds, const float (&pb)[16]
 // only: vector-memory loads retire in order among themselves, and the
 // stores issued between them (the previous tile's xbar) can only lower
 // the count, so vmcnt(4) still means this tile's loads are done.
+template <bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_bwd64d(const float* __restrict__ x, const float* __restrict__ ybar,
              float* __restrict__ xbar, const Geom g, const uint32_t n_orbits,
@@ -623,11 +629,11 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t ntiles = n_orbits / kDirTile, stride = gridDim.x;
   uint32_t t = blockIdx.x;
   int n_acc = 0;
-  if (t < ntiles) dir_issue(x, ybar, g, t, w, c, oin, oout, st0);
+  if (t < ntiles) dir_issue<NT>(x, ybar, g, t, w, c, oin, oout, st0);
   while (t < ntiles) {
     const uint32_t t1 = t + stride;
     if (t1 < ntiles) {
-      dir_issue(x, ybar, g, t1, w, c, oin, oout, st1);
+      dir_issue<NT>(x, ybar, g, t1, w, c, oin, oout, st1);
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
     } else {
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
@@ -640,7 +646,7 @@ __global__ void __launch_bounds__(kBlock)
     if (t1 >= ntiles) break;
     const uint32_t t2 = t1 + stride;
     if (t2 < ntiles) {
-      dir_issue(x, ybar, g, t2, w, c, oin, oout, st0);
+      dir_issue<NT>(x, ybar, g, t2, w, c, oin, oout, st0);
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
     } else {
       __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
@@ -1180,8 +1186,13 @@ static void launch_grad(const GradPlan& p, int vn, bool quad, dim3 grid, const v
   if (grad_wide(p)) {
     if constexpr (sizeof(T) == 4) {
       if (grad64_direct(p, 4, vn)) {
-        hipLaunchKernelGGL(k_ptm_grad64d, grid, dim3(kBlock), 0, s, xx, yy, p.g, p.offs,
-                           p.n_orbits, partial);
+        const bool nt = (p.n_in + p.n_out) * 4 * int64_t(grid.y) > nt_threshold_bytes();
+        if (nt)
+          hipLaunchKernelGGL(k_ptm_grad64d<true>, grid, dim3(kBlock), 0, s, xx, yy, p.g, p.offs,
+                             p.n_orbits, partial);
+        else
+          hipLaunchKernelGGL(k_ptm_grad64d<false>, grid, dim3(kBlock), 0, s, xx, yy, p.g,
+                             p.offs, p.n_orbits, partial);
         return;
       }
     }
@@ -1949,7 +1960,7 @@ static int bwd64d_blocks(const GradPlan& p, int batch) {
         cus <= 0)
       cus = 256;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(k_bwd64d), kBlock, 0) != hipSuccess ||
+            &per_cu, reinterpret_cast<const void*>(k_bwd64d<true>), kBlock, 0) != hipSuccess ||
         per_cu <= 0)
       per_cu = 1;
     (void)hipGetLastError();
@@ -2010,12 +2021,15 @@ extern "C" int qsv_apply_ptm_batched_backward(int dtype, const void* x, const vo
                 (long long)need);
   double* partial = static_cast<double*>(workspace);
   const dim3 grid(nb, batch);
-  if (direct)
-    hipLaunchKernelGGL(k_bwd64d, grid, dim3(kBlock), 0, s, static_cast<const float*>(x),
+  if (direct) {
+    // x and ybar read, xbar written: the apply's nontemporal rule
+    const bool nt = (2 * p.n_in + p.n_out) * 4 * int64_t(batch) > nt_threshold_bytes();
+    auto kern = nt ? k_bwd64d<true> : k_bwd64d<false>;
+    hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, s, static_cast<const float*>(x),
                        static_cast<const float*>(ybar), static_cast<float*>(xbar), p.g,
                        p.n_orbits, static_cast<const float*>(ptm_dev), ptm_batch_stride, p.offs,
                        partial);
-  else
+  } else
     hipLaunchKernelGGL((k_bwd64<double, 2>), grid, dim3(kBlock), 0, s,
                        static_cast<const double*>(x), static_cast<const double*>(ybar),
                        static_cast<double*>(xbar), p.g, p.n_orbits,
