@@ -667,6 +667,206 @@ __global__ void __launch_bounds__(kBlock)
     for (int r = 0; r < 4; ++r) out[(16 * w + 4 * k + r) * 64 + 16 * cb + col] = acc64[cb][r];
 }
 
+// One-pass float32 backward of a dense 64 x 64 block whose targets include
+// the innermost axis (entries in aligned, contiguous runs of four: "quad"
+// geometry).  Tile = 32 orbits; wave-load block ks (0..7) holds orbits
+// 4ks..4ks+3, its lane l = 4q + okk taking entries 4q..4q+3 of orbit
+// 4ks + okk (16 bytes; when the targets are the last three axes an orbit's
+// 64 entries are one 256-byte run, so a wave-load reads 1 KB contiguous).
+// Blocks are 1088 bytes apart (64 bytes of padding), which puts every
+// operand read below on distinct banks.  Per tile:
+//   dP += ybar x^T: wave w owns rows 16w..16w+15, k = the tile's orbits
+//     (8 k-steps of 4), one 4-byte read per operand per lane;
+//   xbar = P^T ybar: wave w computes output entries 16w..16w+15 (one row
+//     tile) for both 16-orbit column tiles; k-step s covers entries
+//     16 (s / 4) + 4 kk + s % 4 (k_fused64's grouping for 16-byte runs, so
+//     xbar is bitwise the transposed apply's): lane (n, kk) reads a whole
+//     quad of ybar (16 bytes, four k-steps), P's 16 fragments per lane
+//     stay in registers, and accumulator slots r of lane (n, j) are the
+//     quad 16w + 4j + r of orbit n: one 16-byte store per column tile.
+constexpr int kQBlk = 1088;          // bytes per wave-load block (1024 + pad)
+constexpr int kQArr = 8 * kQBlk;     // bytes per operand tile
+
+template <bool NT>
+__device__ __forceinline__ void quad_issue(const float* x, const float* ybar, const Geom& g,
+                                           uint32_t t, int w, int lane, int64_t oiq,
+                                           int64_t ooq, char* lds) {
+  constexpr int kPol = NT ? 2 : 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int ks = 2 * w + h;
+    int64_t bin, bout;
+    orbit_base(g, t * kDirTile + 4 * ks + (lane & 3), bin, bout);
+    __builtin_amdgcn_global_load_lds(x + bin + oiq, lds + ks * kQBlk, 16, 0, kPol);
+    __builtin_amdgcn_global_load_lds(ybar + bout + ooq, lds + kQArr + ks * kQBlk, 16, 0,
+                                     kPol);
+  }
+}
+
+// Two k-steps (KS, KS + 1) of the gradient: A (ybar) and the four B (x)
+// operands of each, read and waited for in one asm block.
+#define QSV_Q_READ2(KS)                                                            \
+  asm volatile(                                                                    \
+      "ds_read_b32 %0, %10 offset:%c12\n"                                          \
+      "ds_read_b32 %1, %11 offset:%c12\n"                                          \
+      "ds_read_b32 %2, %11 offset:%c13\n"                                          \
+      "ds_read_b32 %3, %11 offset:%c14\n"                                          \
+      "ds_read_b32 %4, %11 offset:%c15\n"                                          \
+      "ds_read_b32 %5, %10 offset:%c16\n"                                          \
+      "ds_read_b32 %6, %11 offset:%c16\n"                                          \
+      "ds_read_b32 %7, %11 offset:%c17\n"                                          \
+      "ds_read_b32 %8, %11 offset:%c18\n"                                          \
+      "ds_read_b32 %9, %11 offset:%c19\n"                                          \
+      "s_waitcnt lgkmcnt(0)\n"                                                     \
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]),          \
+        "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7]), "=&v"(v[8]), "=&v"(v[9])            \
+      : "v"(ya), "v"(xb), "i"(kQBlk * (KS)), "i"(kQBlk * (KS) + 256),             \
+        "i"(kQBlk * (KS) + 512), "i"(kQBlk * (KS) + 768), "i"(kQBlk * (KS + 1)),   \
+        "i"(kQBlk * (KS + 1) + 256), "i"(kQBlk * (KS + 1) + 512),                 \
+        "i"(kQBlk * (KS + 1) + 768)                                                \
+      : "memory")
+
+__device__ __forceinline__ void quad_grad(const char* lds, int w, int lane,
+                                          Mfma<float>::acc (&acc)[4]) {
+  const int m = lane & 15, kk = lane >> 4;
+  const int row = 64 * (m >> 2) + 16 * kk + 4 * (m & 3);
+  const uint32_t ya = lds_off(lds + kQArr + 256 * w + row);
+  const uint32_t xb = lds_off(lds + row);
+  float v[10];
+#define QSV_Q_STEP2(KS)                                         \
+  QSV_Q_READ2(KS);                                              \
+  _Pragma("unroll") for (int h = 0; h < 2; ++h)                 \
+  _Pragma("unroll") for (int cb = 0; cb < 4; ++cb)              \
+    acc[cb] = Mfma<float>::op(v[5 * h], v[5 * h + 1 + cb], acc[cb]);
+  QSV_Q_STEP2(0)
+  QSV_Q_STEP2(2)
+  QSV_Q_STEP2(4)
+  QSV_Q_STEP2(6)
+#undef QSV_Q_STEP2
+}
+
+typedef float qsv_f4 __attribute__((ext_vector_type(4)));
+
+// The ybar quads of one column tile (four groups of four k-steps).
+#define QSV_Q_READQ(CT)                                                            \
+  asm volatile(                                                                    \
+      "ds_read_b128 %0, %4 offset:%c5\n"                                           \
+      "ds_read_b128 %1, %4 offset:%c6\n"                                           \
+      "ds_read_b128 %2, %4 offset:%c7\n"                                           \
+      "ds_read_b128 %3, %4 offset:%c8\n"                                           \
+      "s_waitcnt lgkmcnt(0)\n"                                                     \
+      : "=&v"(bq[0]), "=&v"(bq[1]), "=&v"(bq[2]), "=&v"(bq[3])                     \
+      : "v"(yb), "i"(4 * kQBlk * (CT)), "i"(4 * kQBlk * (CT) + 256),              \
+        "i"(4 * kQBlk * (CT) + 512), "i"(4 * kQBlk * (CT) + 768)                   \
+      : "memory")
+
+__device__ __forceinline__ void quad_apply(const char* lds, const float (&pa)[16], int lane,
+                                           const Geom& g, uint32_t t, int64_t xo,
+                                           float* __restrict__ xbar) {
+  const int n = lane & 15, kk = lane >> 4;
+  const uint32_t yb = lds_off(lds + kQArr + (n >> 2) * kQBlk + 64 * kk + 16 * (n & 3));
+  typedef Mfma<float>::acc Acc;
+  qsv_f4 bq[4];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    if (ct == 0) {
+      QSV_Q_READQ(0);
+    } else {
+      QSV_Q_READQ(1);
+    }
+    Acc a = Acc(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a = Mfma<float>::op(pa[4 * u + e], bq[u][e], a);
+    int64_t bin, bout;
+    orbit_base(g, t * kDirTile + 16 * ct + n, bin, bout);
+    *reinterpret_cast<qsv_f4*>(xbar + bin + xo) = a;
+  }
+}
+
+template <bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_bwd64q(const float* __restrict__ x, const float* __restrict__ ybar,
+             float* __restrict__ xbar, const Geom g, const uint32_t n_orbits,
+             const float* __restrict__ ptm, const int64_t pstride, const GradOffs offs,
+             double* __restrict__ partial) {
+  __shared__ __align__(16) char st0[2 * kQArr];
+  __shared__ __align__(16) char st1[2 * kQArr];
+  const uint32_t b = blockIdx.y;
+  x += int64_t(b) * g.bs_in;
+  xbar += int64_t(b) * g.bs_in;
+  ybar += int64_t(b) * g.bs_out;
+  ptm += int64_t(b) * pstride;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t oiq = offs.off_in[4 * (lane >> 2)], ooq = offs.off_out[4 * (lane >> 2)];
+  const int m = lane & 15, kk = lane >> 4;
+  // A of the adjoint apply: P^T[16w + m][k] = P[k][16w + m], k-step s at
+  // entry 16 (s / 4) + 4 kk + s % 4
+  float pa[16];
+#pragma unroll
+  for (int sk = 0; sk < 16; ++sk)
+    pa[sk] = ptm[(16 * (sk >> 2) + 4 * kk + (sk & 3)) * 64 + 16 * w + m];
+  const int64_t xo = offs.off_in[16 * w + 4 * kk];
+  __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // the PTM, before any tile load
+  typedef Mfma<float>::acc Acc;
+  Acc acc[4];
+  double acc64[4][4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    acc[cb] = Acc(0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc64[cb][r] = 0.0;
+  }
+  auto flush = [&]() {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[cb][r] += double(acc[cb][r]);
+      acc[cb] = Acc(0);
+    }
+  };
+  const uint32_t ntiles = n_orbits / kDirTile, stride = gridDim.x;
+  uint32_t t = blockIdx.x;
+  int n_acc = 0;
+  if (t < ntiles) quad_issue<NT>(x, ybar, g, t, w, lane, oiq, ooq, st0);
+  while (t < ntiles) {
+    const uint32_t t1 = t + stride;
+    if (t1 < ntiles) {
+      quad_issue<NT>(x, ybar, g, t1, w, lane, oiq, ooq, st1);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    lds_barrier();
+    quad_grad(st0, w, lane, acc);
+    quad_apply(st0, pa, lane, g, t, xo, xbar);
+    if (++n_acc == kDirFlush) { flush(); n_acc = 0; }
+    lds_barrier();
+    if (t1 >= ntiles) break;
+    const uint32_t t2 = t1 + stride;
+    if (t2 < ntiles) {
+      quad_issue<NT>(x, ybar, g, t2, w, lane, oiq, ooq, st0);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(4));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    lds_barrier();
+    quad_grad(st1, w, lane, acc);
+    quad_apply(st1, pa, lane, g, t1, xo, xbar);
+    if (++n_acc == kDirFlush) { flush(); n_acc = 0; }
+    lds_barrier();
+    t = t2;
+  }
+  flush();
+  const int col = lane & 15, k = lane >> 4;
+  double* out = partial + (int64_t(b) * gridDim.x + blockIdx.x) * 64 * 64;
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[(16 * w + 4 * k + r) * 64 + 16 * cb + col] = acc64[cb][r];
+}
+
 // PTMs above 64 x 64 (qutrit pairs 81 x 81, ...): target dims and strides,
 // the entry offsets decoded by the kernel (mixed radix, last target fastest).
 constexpr int kGradBigMaxSide = 256;
@@ -796,6 +996,12 @@ constexpr int kGrad64sTile = 16;
 static bool grad64_direct(const GradPlan& p, int elem_bytes, int vn) {
   return elem_bytes == 4 && !p.big && p.din == 64 && p.dout == 64 && vn == 4 &&
          p.n_orbits % kDirTile == 0;
+}
+
+// Float32 64 x 64 whose entries come in aligned runs of four (the innermost
+// axis a target) and whole 32-orbit tiles: the one-pass backward k_bwd64q.
+static bool bwd64q_applies(const GradPlan& p) {
+  return !p.big && p.din == 64 && p.dout == 64 && p.quad && p.n_orbits % kDirTile == 0;
 }
 
 static bool grad64_staged(const GradPlan& p, int elem_bytes, int vn) {
@@ -1928,9 +2134,10 @@ static int bwd64_plan(int dtype, int rank, const int64_t* dims, int k, const int
   }
   const int rc = make_grad_plan(dtype_size(dtype), rank, dims, k, targets, dout, p);
   if (rc) return rc;
+  if (dtype == QSV_F32 && bwd64q_applies(p)) return QSV_OK;
   if (!bwd64_applies(p)) {
     set_error("fused backward needs a square 64 x 64 PTM and a unit-stride bystander "
-              "innermost axis");
+              "innermost axis (float32: or a target innermost axis of dimension 4)");
     return QSV_ENOTSUP;
   }
   if (dtype == QSV_F32 && !grad64_direct(p, 4, p.vn)) {
@@ -1952,20 +2159,25 @@ static int bwd64_blocks(const GradPlan& p, int batch, uint32_t cap) {
 // k_bwd64d: one round of resident blocks over the batch (occupancy x CUs /
 // batch per state; measured on 8-12-qubit states: 5-12 % faster than the
 // gradient's 512 orbits per block), at least 4 tiles per block.
+static uint32_t resident_blocks(const void* kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  (void)hipGetLastError();
+  return uint32_t(per_cu) * uint32_t(cus);
+}
+
 static int bwd64d_blocks(const GradPlan& p, int batch) {
-  static const uint32_t cap = [] {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void*>(k_bwd64d<true>), kBlock, 0) != hipSuccess ||
-        per_cu <= 0)
-      per_cu = 1;
-    (void)hipGetLastError();
-    return uint32_t(per_cu) * uint32_t(cus);
-  }();
+  static const uint32_t cap_d =
+      resident_blocks(reinterpret_cast<const void*>(k_bwd64d<true>));
+  static const uint32_t cap_q =
+      resident_blocks(reinterpret_cast<const void*>(k_bwd64q<true>));
+  const uint32_t cap = p.quad ? cap_q : cap_d;
   const uint32_t tiles = p.n_orbits / kDirTile;
   uint32_t nb = cap / uint32_t(std::max(batch, 1));
   nb = std::min(nb, tiles / 4);
@@ -2014,7 +2226,8 @@ extern "C" int qsv_apply_ptm_batched_backward(int dtype, const void* x, const vo
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   const FusedOffs fo = bwd64_offsets(p);
-  const bool direct = dtype == QSV_F32;  // bwd64_plan admits only k_bwd64d's geometries
+  // bwd64_plan admits only k_bwd64d's and k_bwd64q's float32 geometries
+  const bool direct = dtype == QSV_F32;
   const int nb = direct ? bwd64d_blocks(p, batch) : bwd64_blocks(p, batch, bwd64_cap<double, 2>());
   const int64_t need = int64_t(batch) * nb * 64 * 64 * 8;
   QSV_CHECK_ARG(workspace && workspace_bytes >= need, "fused backward needs %lld workspace bytes",
@@ -2024,7 +2237,8 @@ extern "C" int qsv_apply_ptm_batched_backward(int dtype, const void* x, const vo
   if (direct) {
     // x and ybar read, xbar written: the apply's nontemporal rule
     const bool nt = (2 * p.n_in + p.n_out) * 4 * int64_t(batch) > nt_threshold_bytes();
-    auto kern = nt ? k_bwd64d<true> : k_bwd64d<false>;
+    auto kern = p.quad ? (nt ? k_bwd64q<true> : k_bwd64q<false>)
+                       : (nt ? k_bwd64d<true> : k_bwd64d<false>);
     hipLaunchKernelGGL(kern, grid, dim3(kBlock), 0, s, static_cast<const float*>(x),
                        static_cast<const float*>(ybar), static_cast<float*>(xbar), p.g,
                        p.n_orbits, static_cast<const float*>(ptm_dev), ptm_batch_stride, p.offs,
@@ -2047,12 +2261,154 @@ extern "C" int qsv_apply_ptm_batched_backward(int dtype, const void* x, const vo
   return QSV_OK;
 }
 
+// Host replay of k_bwd64q + k_ptm_grad_final for one state: every direct
+// load (bounds, alignment, its four entries contiguous, its LDS slot),
+// every operand read of both contractions finding its (entry, orbit) in
+// the current stage, the MFMAs in hardware index order, every xbar store
+// in bounds and each xbar entry written once, each operand loaded once.
+static int emulate_bwd64q(const GradPlan& p, const double* x, const double* ybar, int nb,
+                          double* dptm, const double* ptm, double* xbar) {
+  QSV_CHECK_ARG(bwd64q_applies(p), "bwd64q: not a quad geometry");
+  const uint32_t ntiles = p.n_orbits / kDirTile;
+  const int nf = 2 * kQArr / 4;
+  std::vector<double> sv(nf);
+  std::vector<int64_t> stg(size_t(nf) * 4);
+  std::vector<int> loaded(size_t(64) * p.n_orbits * 2, 0), xwritten(p.n_in, 0);
+  std::vector<double> partial(size_t(nb) * 4096, 0.0);
+  for (int blk = 0; blk < nb; ++blk) {
+    std::vector<double> D(4 * 4 * 16 * 16, 0.0);  // [w][cb][row][col]
+    for (uint32_t t = blk; t < ntiles; t += nb) {
+      std::fill(stg.begin(), stg.end(), int64_t(-1));
+      for (int w = 0; w < kBlock / 64; ++w)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int q = lane >> 2;
+          for (int h = 0; h < 2; ++h) {
+            const int ks = 2 * w + h;
+            const uint32_t orb = t * kDirTile + 4 * ks + (lane & 3);
+            int64_t bin, bout;
+            orbit_base(p.g, orb, bin, bout);
+            for (int arr = 0; arr < 2; ++arr) {
+              const int64_t* off = arr == 0 ? p.offs.off_in : p.offs.off_out;
+              const int64_t a = (arr == 0 ? bin : bout) + off[4 * q];
+              const int64_t n_tot = arr == 0 ? p.n_in : p.n_out;
+              QSV_CHECK_ARG(a >= 0 && a + 4 <= n_tot && a % 4 == 0,
+                            "bwd64q: direct load out of bounds or misaligned");
+              for (int r = 1; r < 4; ++r)
+                QSV_CHECK_ARG(off[4 * q + r] == off[4 * q] + r, "bwd64q: quad not contiguous");
+              const int byte = arr * kQArr + ks * kQBlk + 16 * lane;
+              QSV_CHECK_ARG(byte >= 0 && byte + 16 <= 2 * kQArr, "bwd64q: LDS slot");
+              for (int r = 0; r < 4; ++r) {
+                const int f = byte / 4 + r;
+                QSV_CHECK_ARG(stg[size_t(f) * 4] == -1, "bwd64q: LDS float written twice");
+                stg[size_t(f) * 4 + 0] = 4 * q + r;
+                stg[size_t(f) * 4 + 1] = orb;
+                stg[size_t(f) * 4 + 2] = arr;
+                stg[size_t(f) * 4 + 3] = t;
+                sv[f] = (arr == 0 ? x : ybar)[a + r];
+                ++loaded[(size_t(orb) * 64 + 4 * q + r) * 2 + arr];
+              }
+            }
+          }
+        }
+      auto fetch = [&](int byte, int ent, uint32_t orb, int arr, double& out) -> int {
+        QSV_CHECK_ARG(byte >= 0 && byte + 4 <= 2 * kQArr && byte % 4 == 0,
+                      "bwd64q: operand read outside the stage");
+        const int f = byte / 4;
+        QSV_CHECK_ARG(stg[size_t(f) * 4] == ent && stg[size_t(f) * 4 + 1] == orb &&
+                          stg[size_t(f) * 4 + 2] == arr && stg[size_t(f) * 4 + 3] == int64_t(t),
+                      "bwd64q: operand read finds the wrong entry or orbit");
+        out = sv[f];
+        return QSV_OK;
+      };
+      for (int w = 0; w < kBlock / 64; ++w) {
+        // dP: k-step ks over orbits 4ks + kk
+        for (int ks = 0; ks < 8; ++ks)
+          for (int kk = 0; kk < 4; ++kk) {
+            const uint32_t orb = t * kDirTile + 4 * ks + kk;
+            double A[16], B[4][16];
+            for (int mm = 0; mm < 16; ++mm) {
+              const int row = 64 * (mm >> 2) + 16 * kk + 4 * (mm & 3) + ks * kQBlk;
+              int rc = fetch(kQArr + 256 * w + row, 16 * w + mm, orb, 1, A[mm]);
+              if (rc) return rc;
+              for (int cb = 0; cb < 4; ++cb) {
+                rc = fetch(row + 256 * cb, 16 * cb + mm, orb, 0, B[cb][mm]);
+                if (rc) return rc;
+              }
+            }
+            for (int cb = 0; cb < 4; ++cb)
+              for (int r = 0; r < 16; ++r)
+                for (int cc = 0; cc < 16; ++cc)
+                  D[((w * 4 + cb) * 16 + r) * 16 + cc] += A[r] * B[cb][cc];
+          }
+        if (!xbar) continue;
+        // xbar: column tile ct, k-step s = 4u + e at entry 16u + 4kk + e
+        for (int ct = 0; ct < 2; ++ct) {
+          double X[16][16] = {};  // [row m][column n]
+          for (int u = 0; u < 4; ++u)
+            for (int kk = 0; kk < 4; ++kk)
+              for (int nn = 0; nn < 16; ++nn) {
+                const int byte = kQArr + (nn >> 2) * kQBlk + 64 * kk + 16 * (nn & 3) +
+                                 4 * kQBlk * ct + 256 * u;
+                QSV_CHECK_ARG(byte % 16 == 0, "bwd64q: 16-byte read misaligned");
+                for (int e = 0; e < 4; ++e) {
+                  const int ent = 16 * u + 4 * kk + e;
+                  double bv;
+                  const int rc =
+                      fetch(byte + 4 * e, ent, t * kDirTile + 16 * ct + nn, 1, bv);
+                  if (rc) return rc;
+                  for (int mm = 0; mm < 16; ++mm) X[mm][nn] += ptm[ent * 64 + 16 * w + mm] * bv;
+                }
+              }
+          // slot r of lane (n, j): row 4j + r = entry 16w + 4j + r of orbit n
+          for (int lane = 0; lane < 64; ++lane) {
+            const int nn = lane & 15, j = lane >> 4;
+            int64_t bin, bout;
+            orbit_base(p.g, t * kDirTile + 16 * ct + nn, bin, bout);
+            const int64_t a = bin + p.offs.off_in[16 * w + 4 * j];
+            QSV_CHECK_ARG(a >= 0 && a + 4 <= p.n_in && a % 4 == 0,
+                          "bwd64q: xbar store out of bounds or misaligned");
+            for (int r = 0; r < 4; ++r) {
+              QSV_CHECK_ARG(p.offs.off_in[16 * w + 4 * j + r] == p.offs.off_in[16 * w + 4 * j] + r,
+                            "bwd64q: xbar quad not contiguous");
+              xbar[a + r] = X[4 * j + r][nn];
+              ++xwritten[a + r];
+            }
+          }
+        }
+      }
+    }
+    std::vector<int> written(4096, 0);
+    for (int w = 0; w < kBlock / 64; ++w)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int cb = 0; cb < 4; ++cb)
+          for (int r = 0; r < 4; ++r) {
+            const int row = 4 * (lane >> 4) + r, col = lane & 15;
+            const int idx = (16 * w + row) * 64 + 16 * cb + col;
+            ++written[idx];
+            partial[size_t(blk) * 4096 + idx] = D[((w * 4 + cb) * 16 + row) * 16 + col];
+          }
+    for (int i = 0; i < 4096; ++i)
+      QSV_CHECK_ARG(written[i] == 1, "bwd64q: partial entry written %d times", written[i]);
+  }
+  for (size_t i = 0; i < loaded.size(); ++i)
+    QSV_CHECK_ARG(loaded[i] == 1, "bwd64q: operand loaded %d times", loaded[i]);
+  for (size_t i = 0; i < xwritten.size(); ++i)
+    QSV_CHECK_ARG(xwritten[i] == 1, "bwd64q: xbar entry %lld written %d times", (long long)i,
+                  xwritten[i]);
+  for (int j = 0; j < 4096; ++j) {
+    double sum = 0.0;
+    for (int b = 0; b < nb; ++b) sum += partial[size_t(b) * 4096 + j];
+    dptm[j] = sum;
+  }
+  return QSV_OK;
+}
+
 // Host replay of k_bwd64 + k_ptm_grad_final for one state.  Part A: every
 // lane's loads and stores (bounds, vector alignment, each xbar entry written
 // once) and the MFMA contraction in hardware index order with the
 // transposed PTM's fragment table; part B: the wide-gradient replay with
-// the kernel's four k-steps per tile.  Float32 geometries of k_bwd64d
-// replay that kernel instead (emulate_grad64d with its apply half).
+// the kernel's four k-steps per tile.  Float32 geometries replay k_bwd64d
+// (emulate_grad64d with its apply half) or k_bwd64q instead.
 extern "C" int qsv_emulate_apply_ptm_batched_backward(int dtype, const double* x,
                                                       const double* ybar, double* xbar,
                                                       int rank, const int64_t* dims, int k,
@@ -2067,10 +2423,11 @@ extern "C" int qsv_emulate_apply_ptm_batched_backward(int dtype, const double* x
   QSV_CHECK_ARG(grad_blocks >= 0 && grad_blocks <= kGrad64Blocks, "grad_blocks outside [0, %d]",
                 kGrad64Blocks);
   const int es = dtype_size(dtype), VN = p.vn, MB = 4, KS = 16;
-  if (dtype == QSV_F32)  // bwd64_plan admits only k_bwd64d's geometries
-    return emulate_grad64d(p, x, ybar,
-                           grad_blocks > 0 ? grad_blocks : grad64_blocks(p.n_orbits, 1), dptm,
-                           ptm, xbar);
+  if (dtype == QSV_F32) {  // bwd64_plan admits only k_bwd64d's and k_bwd64q's geometries
+    const int nbd = grad_blocks > 0 ? grad_blocks : grad64_blocks(p.n_orbits, 1);
+    return p.quad ? emulate_bwd64q(p, x, ybar, nbd, dptm, ptm, xbar)
+                  : emulate_grad64d(p, x, ybar, nbd, dptm, ptm, xbar);
+  }
   const int nb = grad_blocks > 0 ? grad_blocks : bwd64_blocks(p, 1, 256);
   const FusedOffs fo = bwd64_offsets(p);
   // fragment table of P^T, as the kernel gathers it

@@ -463,14 +463,16 @@ def test_trainable_circuit_fused_gradient():
 @pytest.mark.parametrize('n,targets', [(6, (1, 2, 3)), (6, (0, 1, 2)), (6, (3, 0, 2)),
                                        (10, (4, 5, 6)), (10, (7, 8, 9)), (5, (2, 3, 4)),
                                        (9, (0, 1, 2)), (8, (3, 4, 5)), (10, (0, 4, 8)),
-                                       (7, (6, 1, 3))])
+                                       (7, (6, 1, 3)), (6, (3, 4, 5)), (9, (6, 7, 8)),
+                                       (8, (1, 4, 7)), (10, (2, 5, 9)), (7, (6, 3, 4))])
 def test_fused_backward_matches_separate_kernels(n, targets, dtype, tol):
     """One-pass backward of dense 64 x 64 blocks (qsv_apply_ptm_batched_backward)
     vs the transposed apply (bitwise: the same k-steps of four entries in
-    the same order; float32 states with the innermost axis a bystander and
-    whole 32-orbit tiles run the direct-to-LDS k_bwd64d) and the PTM
-    gradient kernel; per-state and shared PTMs; geometries where it does not
-    apply (innermost axis a target) fall back to the separate kernels."""
+    the same order; float32 states on whole 32-orbit tiles run the
+    direct-to-LDS k_bwd64d (innermost axis a bystander) or k_bwd64q (the
+    last target innermost)) and the PTM gradient kernel; per-state and
+    shared PTMs; geometries where it does not apply fall back to the
+    separate kernels."""
     from quantumsim_amd.autograd import _backward_raw, _apply_raw, _ptm_grad_raw
     rng = np.random.default_rng(n + sum(targets))
     B = 3

@@ -408,13 +408,46 @@ def test_emulated_fused_backward_direct_matches_oracle(dims, targets, rng):
         assert np.allclose(dp, ref_dp, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
 
 
+@pytest.mark.parametrize('dims,targets', [([4] * 6, (3, 4, 5)), ([4] * 6, (0, 2, 5)),
+                                          ([4] * 7, (4, 5, 6)), ([4] * 7, (1, 3, 6)),
+                                          ([2, 4, 4, 4, 4, 4], (3, 4, 5)),
+                                          ([4, 4, 4, 4, 2, 4, 4, 4], (5, 6, 7))])
+def test_emulated_fused_backward_quad_matches_oracle(dims, targets, rng):
+    """Float32 fused backward of blocks whose last target is the innermost
+    axis (entries in contiguous runs of four; k_bwd64q), replayed: every
+    direct load in bounds with its four entries contiguous, every operand
+    read of both contractions finding its (entry, orbit) in the current
+    stage, every xbar store in bounds and each xbar entry written once;
+    both results against numpy for several block counts."""
+    n = len(dims)
+    x = rng.standard_normal(dims)
+    ybar = rng.standard_normal(dims)
+    P = rng.standard_normal((4,) * 6)
+    assert emulate_grad(nat.QSV_F32, x, ybar, targets, [4, 4, 4])[1] == 1 | 256  # quad
+    ref_xbar = oracle_apply(ybar, np.ascontiguousarray(
+        P.reshape(64, 64).T).reshape((4,) * 6), targets)
+    xl, yl = list(range(n)), list(range(n))
+    for j, t in enumerate(targets):
+        yl[t] = n + j
+    ref_dp = np.einsum(ybar, yl, x, xl, [n + j for j in range(3)] + list(targets)).reshape(64, 64)
+    for blocks in (0, 1, 3, 64):
+        xbar, dp = emulate_backward(nat.QSV_F32, x, ybar, P, targets, blocks)
+        assert np.allclose(xbar, ref_xbar, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
+        assert np.allclose(dp, ref_dp, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
+    # float64 keeps the separate kernels there
+    assert emulate_backward(nat.QSV_F64, x, ybar, P, targets) is None
+
+
 def test_fused_backward_not_applicable():
-    """Innermost axis a target (or a non-64 PTM): QSV_ENOTSUP, callers fall
+    """Partial 32-orbit tiles in float32, an innermost target in float64, or
+    a lowest PTM index off the innermost axis: QSV_ENOTSUP, callers fall
     back to the separate kernels."""
     x = np.zeros([4] * 5)
     P = np.zeros((4,) * 6)
-    assert emulate_backward(nat.QSV_F32, x, x, P, (2, 3, 4)) is None
+    assert emulate_backward(nat.QSV_F32, x, x, P, (2, 3, 4)) is None   # 16 orbits
     assert emulate_backward(nat.QSV_F64, x, x, P, (0, 1, 4)) is None
+    x = np.zeros([4] * 6)
+    assert emulate_backward(nat.QSV_F32, x, x, P, (5, 3, 4)) is None   # lowest index not innermost
 
 
 # ------------------------------------------- PTMs above 64 x 64 (qudit pairs)
