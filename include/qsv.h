@@ -303,10 +303,12 @@ int64_t qsv_ptm_grad_noalloc_scratch_bytes(void);
  * xbar (three streams instead of four).  xbar is bitwise the transposed
  * apply's (the same k-steps of four entries in the same order).  workspace:
  * qsv_ptm_grad_workspace_bytes of the same geometry.  QSV_ENOTSUP unless the
- * innermost axis is a unit-stride bystander and the states are 16-byte
- * aligned, and for float32 unless the orbits come in 4-orbit vectors and
- * whole 32-orbit tiles (the direct-to-LDS kernel; callers then use
- * qsv_apply_ptm_batched with the transposed PTM and qsv_ptm_grad).  xbar
+ * states are 16-byte aligned and, for float64, the innermost axis is a
+ * unit-stride bystander; for float32, the tiles of 32 orbits are whole and
+ * either the orbits come in 4-orbit vectors (innermost axis a bystander)
+ * or the PTM's lowest index is the innermost axis (entries in runs of
+ * four): the two direct-to-LDS kernels.  Otherwise callers use
+ * qsv_apply_ptm_batched with the transposed PTM and qsv_ptm_grad.  xbar
  * must not alias x or ybar.  No counterpart in the reference; the adjoint
  * of the differentiable and batched paths. */
 int qsv_apply_ptm_batched_backward(int dtype, const void* x, const void* ybar, void* xbar,
