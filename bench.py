@@ -304,10 +304,12 @@ def bench_batched(args):
                         'unit': 'GB/s', 'frac': round(alg_bytes / sec / 1e9 / HBM_PEAK_GBS, 4),
                         'per_kernel': per_kernel,
                         'note': 'alg bytes: every library launch of one step ((N_in + N_out) '
-                                's per apply, 2 N s + dP per PTM gradient, trajectories of the '
-                                'persistent kernels); achieved = bytes / graph-replay step '
-                                'time; per_kernel ms: events around each launch of one eager '
-                                'step, the device idle before and after it'},
+                                's per apply, 2 N s + dP per PTM gradient, 3 N s + dP per '
+                                'one-pass backward, trajectories of the persistent kernels); '
+                                'achieved = bytes / graph-replay step time; per_kernel ms: '
+                                'events around each launch of one eager step, the device idle '
+                                'before and after it; the persistent kernels keep their states '
+                                'in LDS, so their HBM fraction is not their bound'},
            'dtype': 'f32 state, f64 readout',
            'checkpoint_every': cb.k, 'kept_state_gb': round(cb.kept_bytes(cb.k) / 1e9, 1),
            'checkpoint_budget_gb': round(budget / 1e9, 1),
