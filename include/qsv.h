@@ -425,6 +425,16 @@ int qsv_emulate_apply_ptm_batched_backward(int dtype, const double* x, const dou
                                            const int32_t* targets, const double* ptm,
                                            double* dptm, int32_t grad_blocks);
 
+/* CPU replay of the direct-to-LDS float32 64 x 64 apply of the batched and
+ * device-PTM paths (k_apply64q: the PTM's lowest index on the innermost
+ * axis, whole 32-orbit tiles) for one state (host float64 arrays; ptm
+ * row-major 64 x 64; blocks 0 = one block per four tiles): every load,
+ * LDS read and store checked, y written once per entry.  QSV_ENOTSUP for
+ * other geometries.  No counterpart in the reference. */
+int qsv_emulate_apply_ptm_direct(int dtype, const double* x, double* y, int rank,
+                                 const int64_t* dims, int k, const int32_t* targets,
+                                 const double* ptm, int32_t blocks);
+
 /* qsv_emulate_apply_ptm of the batched path's kernel choice for one state,
  * with an explicit launch shape (the autotuner's candidates are replayed
  * with it).  QSV_ENOTSUP when the tile is infeasible for the geometry. */

@@ -83,6 +83,9 @@ _SIGNATURES = {
                                                               _c_f64p, ctypes.c_int, _c_i64p,
                                                               ctypes.c_int, _c_i32p, _c_f64p,
                                                               _c_f64p, ctypes.c_int32]),
+    'qsv_emulate_apply_ptm_direct': (ctypes.c_int, [ctypes.c_int, _c_f64p, _c_f64p, ctypes.c_int,
+                                                    _c_i64p, ctypes.c_int, _c_i32p, _c_f64p,
+                                                    ctypes.c_int32]),
     'qsv_window_blocks': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                          _vp, _vp, ctypes.c_int64, _vp]),

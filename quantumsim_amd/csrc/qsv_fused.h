@@ -117,6 +117,12 @@ int launch_fused64_batched(int dtype, const FusedPlan& fp, const Geom& g, bool n
                            const void* in, void* out, const void* ptm_rows, int64_t pstride,
                            uint32_t batch, hipStream_t s);
 
+// Float32 64 x 64 whose PTM's lowest index is the innermost axis (runs of
+// four entries) on whole 32-orbit tiles: the direct-to-LDS apply.
+bool apply64q_applies(int elem_bytes, const FusedPlan& fp);
+int launch_apply64q(const FusedPlan& fp, const Geom& g, bool nt, const void* in, void* out,
+                    const void* ptm_rows, int64_t pstride, uint32_t batch, hipStream_t s);
+
 // Packs a device PTM (row-major D x D, state's element type) into fragment
 // order at pf (D * D elements) on stream s; the device form of
 // fused_fragments.

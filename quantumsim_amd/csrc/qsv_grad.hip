@@ -22,6 +22,7 @@
 #include "qsv_lds.h"
 #include "qsv_mfma.h"
 
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <mutex>
@@ -865,6 +866,141 @@ __global__ void __launch_bounds__(kBlock)
   for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[(16 * w + 4 * k + r) * 64 + 16 * cb + col] = acc64[cb][r];
+}
+
+// Float32 64 x 64 apply y = P x for the quad geometry on direct-loaded
+// tiles: x staged as in k_bwd64q (one operand), wave w computes output
+// entries 16w..16w+15 of both 16-orbit column tiles with the k-steps of
+// k_fused64's runs-of-four grouping (y bitwise k_fused64's), P's 16
+// fragments per lane in registers, one 16-byte store per column tile.
+template <bool NT>
+__device__ __forceinline__ void quad_issue_x(const float* x, const Geom& g, uint32_t t, int w,
+                                             int lane, int64_t oiq, char* lds) {
+  constexpr int kPol = NT ? 2 : 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int ks = 2 * w + h;
+    int64_t bin, bout;
+    orbit_base(g, t * kDirTile + 4 * ks + (lane & 3), bin, bout);
+    __builtin_amdgcn_global_load_lds(x + bin + oiq, lds + ks * kQBlk, 16, 0, kPol);
+  }
+}
+
+__device__ __forceinline__ void quad_apply_fwd(const char* lds, const float (&pa)[16], int lane,
+                                               const Geom& g, uint32_t t, int64_t yo,
+                                               float* __restrict__ y) {
+  const int n = lane & 15, kk = lane >> 4;
+  const uint32_t yb = lds_off(lds + (n >> 2) * kQBlk + 64 * kk + 16 * (n & 3));
+  typedef Mfma<float>::acc Acc;
+  qsv_f4 bq[4];
+#pragma unroll
+  for (int ct = 0; ct < 2; ++ct) {
+    if (ct == 0) {
+      QSV_Q_READQ(0);
+    } else {
+      QSV_Q_READQ(1);
+    }
+    Acc a = Acc(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a = Mfma<float>::op(pa[4 * u + e], bq[u][e], a);
+    int64_t bin, bout;
+    orbit_base(g, t * kDirTile + 16 * ct + n, bin, bout);
+    *reinterpret_cast<qsv_f4*>(y + bout + yo) = a;
+  }
+}
+
+template <bool NT>
+__global__ void __launch_bounds__(kBlock)
+    k_apply64q(const float* __restrict__ x, float* __restrict__ y, const Geom g,
+               const uint32_t n_orbits, const float* __restrict__ ptm, const int64_t pstride,
+               const GradOffs offs) {
+  __shared__ __align__(16) char st0[kQArr];
+  __shared__ __align__(16) char st1[kQArr];
+  const uint32_t b = blockIdx.y;
+  x += int64_t(b) * g.bs_in;
+  y += int64_t(b) * g.bs_out;
+  ptm += int64_t(b) * pstride;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t oiq = offs.off_in[4 * (lane >> 2)];
+  const int m = lane & 15, kk = lane >> 4;
+  float pa[16];
+#pragma unroll
+  for (int sk = 0; sk < 16; ++sk)
+    pa[sk] = ptm[(16 * w + m) * 64 + 16 * (sk >> 2) + 4 * kk + (sk & 3)];
+  const int64_t yo = offs.off_out[16 * w + 4 * kk];
+  __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));  // the PTM, before any tile load
+  const uint32_t ntiles = n_orbits / kDirTile, stride = gridDim.x;
+  uint32_t t = blockIdx.x;
+  if (t < ntiles) quad_issue_x<NT>(x, g, t, w, lane, oiq, st0);
+  while (t < ntiles) {
+    const uint32_t t1 = t + stride;
+    if (t1 < ntiles) {
+      quad_issue_x<NT>(x, g, t1, w, lane, oiq, st1);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(2));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    lds_barrier();
+    quad_apply_fwd(st0, pa, lane, g, t, yo, y);
+    lds_barrier();
+    if (t1 >= ntiles) break;
+    const uint32_t t2 = t1 + stride;
+    if (t2 < ntiles) {
+      quad_issue_x<NT>(x, g, t2, w, lane, oiq, st0);
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(2));
+    } else {
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    }
+    lds_barrier();
+    quad_apply_fwd(st1, pa, lane, g, t1, yo, y);
+    lds_barrier();
+    t = t2;
+  }
+}
+
+// Resident blocks of `kernel` over the device (occupancy x CUs).
+static uint32_t resident_blocks(const void* kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 1;
+  (void)hipGetLastError();
+  return uint32_t(per_cu) * uint32_t(cus);
+}
+
+// Host side of k_apply64q: natural per-entry offsets from the fused plan's
+// split for runs of four (entry 16u + 4j + r at in_u[u] + in_l[j] + r).
+bool apply64q_applies(int elem_bytes, const FusedPlan& fp) {
+  return elem_bytes == 4 && fp.D == 64 && fp.vi == 4 && fp.vo == 4 && fp.vn == 1 &&
+         fp.n_orbits % kDirTile == 0;
+}
+
+int launch_apply64q(const FusedPlan& fp, const Geom& g, bool nt, const void* in, void* out,
+                    const void* ptm_rows, int64_t pstride, uint32_t batch, hipStream_t s) {
+  GradOffs o;
+  for (int u = 0; u < 4; ++u)
+    for (int j = 0; j < 4; ++j)
+      for (int r = 0; r < 4; ++r) {
+        o.off_in[16 * u + 4 * j + r] = fp.offs.in_u[u] + fp.offs.in_l[j] + r;
+        o.off_out[16 * u + 4 * j + r] = fp.offs.out_u[u] + fp.offs.out_l[j] + r;
+      }
+  static const uint32_t cap =
+      resident_blocks(reinterpret_cast<const void*>(k_apply64q<true>));
+  const uint32_t tiles = fp.n_orbits / kDirTile;
+  uint32_t nb = cap / std::max<uint32_t>(batch, 1);
+  nb = std::max<uint32_t>(1, std::min(nb, std::max<uint32_t>(tiles / 4, 1)));
+  auto kern = nt ? k_apply64q<true> : k_apply64q<false>;
+  hipLaunchKernelGGL(kern, dim3(nb, batch), dim3(kBlock), 0, s, static_cast<const float*>(in),
+                     static_cast<float*>(out), g, fp.n_orbits,
+                     static_cast<const float*>(ptm_rows), pstride, o);
+  QSV_CHECK_HIP(hipGetLastError());
+  return QSV_OK;
 }
 
 // PTMs above 64 x 64 (qutrit pairs 81 x 81, ...): target dims and strides,
@@ -2159,18 +2295,6 @@ static int bwd64_blocks(const GradPlan& p, int batch, uint32_t cap) {
 // k_bwd64d: one round of resident blocks over the batch (occupancy x CUs /
 // batch per state; measured on 8-12-qubit states: 5-12 % faster than the
 // gradient's 512 orbits per block), at least 4 tiles per block.
-static uint32_t resident_blocks(const void* kernel) {
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      cus <= 0)
-    cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
-      per_cu <= 0)
-    per_cu = 1;
-  (void)hipGetLastError();
-  return uint32_t(per_cu) * uint32_t(cus);
-}
 
 static int bwd64d_blocks(const GradPlan& p, int batch) {
   static const uint32_t cap_d =
@@ -2401,6 +2525,119 @@ static int emulate_bwd64q(const GradPlan& p, const double* x, const double* ybar
     dptm[j] = sum;
   }
   return QSV_OK;
+}
+
+// Host replay of k_apply64q for one state: every direct load of x (bounds,
+// alignment, its four entries contiguous, its LDS slot), every 16-byte
+// operand read finding its (entry, orbit) in the current stage, the MFMA
+// contraction in hardware index order, every y store in bounds and each y
+// entry written once, each x entry loaded once.
+static int emulate_apply64q(const GradPlan& p, const double* x, const double* ptm, int nb,
+                            double* y) {
+  QSV_CHECK_ARG(bwd64q_applies(p), "apply64q: not a quad geometry");
+  const uint32_t ntiles = p.n_orbits / kDirTile;
+  const int nf = kQArr / 4;
+  std::vector<double> sv(nf);
+  std::vector<int64_t> stg(size_t(nf) * 3);
+  std::vector<int> loaded(size_t(64) * p.n_orbits, 0), written(p.n_out, 0);
+  for (int blk = 0; blk < nb; ++blk)
+    for (uint32_t t = blk; t < ntiles; t += nb) {
+      std::fill(stg.begin(), stg.end(), int64_t(-1));
+      for (int w = 0; w < kBlock / 64; ++w)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int q = lane >> 2;
+          for (int h = 0; h < 2; ++h) {
+            const int ks = 2 * w + h;
+            const uint32_t orb = t * kDirTile + 4 * ks + (lane & 3);
+            int64_t bin, bout;
+            orbit_base(p.g, orb, bin, bout);
+            const int64_t a = bin + p.offs.off_in[4 * q];
+            QSV_CHECK_ARG(a >= 0 && a + 4 <= p.n_in && a % 4 == 0,
+                          "apply64q: direct load out of bounds or misaligned");
+            const int byte = ks * kQBlk + 16 * lane;
+            QSV_CHECK_ARG(byte >= 0 && byte + 16 <= kQArr, "apply64q: LDS slot");
+            for (int r = 0; r < 4; ++r) {
+              QSV_CHECK_ARG(p.offs.off_in[4 * q + r] == p.offs.off_in[4 * q] + r,
+                            "apply64q: quad not contiguous");
+              const int f = byte / 4 + r;
+              QSV_CHECK_ARG(stg[size_t(f) * 3] == -1, "apply64q: LDS float written twice");
+              stg[size_t(f) * 3 + 0] = 4 * q + r;
+              stg[size_t(f) * 3 + 1] = orb;
+              stg[size_t(f) * 3 + 2] = t;
+              sv[f] = x[a + r];
+              ++loaded[size_t(orb) * 64 + 4 * q + r];
+            }
+          }
+        }
+      for (int w = 0; w < kBlock / 64; ++w)
+        for (int ct = 0; ct < 2; ++ct) {
+          double Y[16][16] = {};  // [row m][column n]
+          for (int u = 0; u < 4; ++u)
+            for (int kk = 0; kk < 4; ++kk)
+              for (int nn = 0; nn < 16; ++nn) {
+                const int byte =
+                    (nn >> 2) * kQBlk + 64 * kk + 16 * (nn & 3) + 4 * kQBlk * ct + 256 * u;
+                QSV_CHECK_ARG(byte % 16 == 0 && byte + 16 <= kQArr,
+                              "apply64q: 16-byte read misaligned or outside the stage");
+                for (int e = 0; e < 4; ++e) {
+                  const int ent = 16 * u + 4 * kk + e, f = byte / 4 + e;
+                  QSV_CHECK_ARG(stg[size_t(f) * 3] == ent &&
+                                    stg[size_t(f) * 3 + 1] == t * kDirTile + 16 * ct + nn &&
+                                    stg[size_t(f) * 3 + 2] == int64_t(t),
+                                "apply64q: operand read finds the wrong entry or orbit");
+                  for (int mm = 0; mm < 16; ++mm) Y[mm][nn] += ptm[(16 * w + mm) * 64 + ent] * sv[f];
+                }
+              }
+          for (int lane = 0; lane < 64; ++lane) {
+            const int nn = lane & 15, j = lane >> 4;
+            int64_t bin, bout;
+            orbit_base(p.g, t * kDirTile + 16 * ct + nn, bin, bout);
+            const int64_t a = bout + p.offs.off_out[16 * w + 4 * j];
+            QSV_CHECK_ARG(a >= 0 && a + 4 <= p.n_out && a % 4 == 0,
+                          "apply64q: y store out of bounds or misaligned");
+            for (int r = 0; r < 4; ++r) {
+              QSV_CHECK_ARG(p.offs.off_out[16 * w + 4 * j + r] == p.offs.off_out[16 * w + 4 * j] + r,
+                            "apply64q: y quad not contiguous");
+              y[a + r] = Y[4 * j + r][nn];
+              ++written[a + r];
+            }
+          }
+        }
+    }
+  for (size_t i = 0; i < loaded.size(); ++i)
+    QSV_CHECK_ARG(loaded[i] == 1, "apply64q: x entry loaded %d times", loaded[i]);
+  for (size_t i = 0; i < written.size(); ++i)
+    QSV_CHECK_ARG(written[i] == 1, "apply64q: y entry %lld written %d times", (long long)i,
+                  written[i]);
+  return QSV_OK;
+}
+
+extern "C" int qsv_emulate_apply_ptm_direct(int dtype, const double* x, double* y, int rank,
+                                            const int64_t* dims, int k, const int32_t* targets,
+                                            const double* ptm, int32_t blocks) {
+  QSV_CHECK_ARG(dtype == QSV_F32 || dtype == QSV_F64, "unknown dtype %d", dtype);
+  QSV_CHECK_ARG(dims && targets && k >= 1 && k <= QSV_MAX_TARGETS && rank >= 1 &&
+                    rank <= QSV_MAX_RANK,
+                "bad shape arguments");
+  int32_t dout[QSV_MAX_TARGETS];
+  for (int j = 0; j < k; ++j) {
+    QSV_CHECK_ARG(targets[j] >= 0 && targets[j] < rank, "bad target %d", targets[j]);
+    dout[j] = int32_t(dims[targets[j]]);
+  }
+  GradPlan p;
+  const int rc = make_grad_plan(dtype_size(dtype), rank, dims, k, targets, dout, p);
+  if (rc) return rc;
+  if (dtype != QSV_F32 || !bwd64q_applies(p)) {
+    set_error("the direct 64 x 64 apply takes float32 states whose PTM's lowest index is the "
+              "innermost axis, on whole %d-orbit tiles", kDirTile);
+    return QSV_ENOTSUP;
+  }
+  QSV_CHECK_ARG(x && y && ptm, "null pointer argument");
+  QSV_CHECK_ARG(p.n_in <= (int64_t(1) << 24), "emulation is for small vectors");
+  QSV_CHECK_ARG(blocks >= 0 && blocks <= 4096, "blocks outside [0, 4096]");
+  const uint32_t tiles = p.n_orbits / kDirTile;
+  const int nb = blocks > 0 ? blocks : int(std::max<uint32_t>(1, tiles / 4));
+  return emulate_apply64q(p, x, ptm, nb, y);
 }
 
 // Host replay of k_bwd64 + k_ptm_grad_final for one state.  Part A: every

@@ -497,6 +497,39 @@ def test_fused_backward_matches_separate_kernels(n, targets, dtype, tol):
         ref = np.einsum(yn, ly, xn, lx, [n + 1] + [n + 2 + j for j in range(3)] + list(targets))
         err = np.max(np.abs(gp.double().cpu().numpy() - ref.reshape(B, 64, 64))) / np.max(np.abs(ref))
         assert err < tol, err
+        # xbar = P^T ybar against numpy (the transposed apply above may run
+        # the same family of direct kernels)
+        Pn = P.double().cpu().numpy().reshape((P.shape[0],) + (4,) * 6)
+        if P.shape[0] == 1:
+            Pn = np.broadcast_to(Pn, (B,) + (4,) * 6)
+        lp = [n + 1] + [n + 2 + j for j in range(3)] + list(targets)
+        xref = np.einsum(Pn, lp, yn, ly, [n + 1] + list(range(n))).reshape(B, -1)
+        err = np.max(np.abs(gx.double().cpu().numpy() - xref)) / np.max(np.abs(xref))
+        assert err < tol, err
+
+
+@pytest.mark.parametrize('n,targets', [(6, (3, 4, 5)), (8, (5, 6, 7)), (10, (0, 4, 9)),
+                                       (9, (2, 7, 8)), (8, (1, 2, 3))])
+def test_batched_apply_dense64_matches_einsum(n, targets):
+    """Float32 batched 64 x 64 applies with per-state PTMs against a float64
+    einsum: the direct-to-LDS kernel where the PTM's lowest index is the
+    innermost axis (k_apply64q), k_fused64 elsewhere."""
+    from quantumsim_amd.autograd import _apply_raw
+    rng = np.random.default_rng(3 * n + targets[0])
+    B = 3
+    shape = (4,) * n
+    print('dense64 apply', n, targets, flush=True)
+    x = rng.standard_normal((B, 4 ** n))
+    P = rng.standard_normal((B, 64, 64)) / 8
+    y = _apply_raw(torch.tensor(x, dtype=torch.float32, device=DEV),
+                   torch.tensor(P, dtype=torch.float32, device=DEV), shape, targets, [4, 4, 4])
+    xs = x.reshape((B,) + shape)
+    lx = [n + 1] + list(range(n))
+    lp = [n + 1] + [n + 2 + j for j in range(3)] + list(targets)
+    lo = [n + 1] + [n + 2 + targets.index(q) if q in targets else q for q in range(n)]
+    ref = np.einsum(P.reshape((B,) + (4,) * 6), lp, xs, lx, lo).reshape(B, -1)
+    err = np.max(np.abs(y.double().cpu().numpy() - ref)) / np.max(np.abs(ref))
+    assert err < 1e-5, err
 
 
 def test_circuit_batch_checkpoint_interval_from_budget():

@@ -438,6 +438,49 @@ def test_emulated_fused_backward_quad_matches_oracle(dims, targets, rng):
     assert emulate_backward(nat.QSV_F64, x, ybar, P, targets) is None
 
 
+@pytest.mark.parametrize('dims,targets', [([4] * 6, (3, 4, 5)), ([4] * 6, (0, 2, 5)),
+                                          ([4] * 7, (4, 5, 6)), ([4] * 7, (1, 3, 6)),
+                                          ([2, 4, 4, 4, 4, 4], (3, 4, 5)),
+                                          ([4, 4, 4, 4, 2, 4, 4, 4], (5, 6, 7))])
+def test_emulated_apply_direct_quad_matches_oracle(dims, targets, rng):
+    """Float32 64 x 64 apply of the batched path when the PTM's lowest index
+    is the innermost axis (k_apply64q), replayed: every direct load of x in
+    bounds with its four entries contiguous, every 16-byte operand read
+    finding its (entry, orbit) in the current stage, every y store in
+    bounds and each y entry written once; y against the oracle for several
+    block counts."""
+    x = rng.standard_normal(dims)
+    P = rng.standard_normal((4,) * 6)
+    ref = oracle_apply(x, P, targets)
+    L = nat.lib()
+    Pf = np.ascontiguousarray(P.reshape(64, 64))
+    xc = np.ascontiguousarray(x)
+    for blocks in (0, 1, 3, 64):
+        y = np.zeros_like(xc)
+        rc = L.qsv_emulate_apply_ptm_direct(nat.QSV_F32, xc.ctypes.data_as(nat._c_f64p),
+                                            y.ctypes.data_as(nat._c_f64p), x.ndim,
+                                            nat.i64(dims), 3, nat.i32(targets),
+                                            Pf.ctypes.data_as(nat._c_f64p), blocks)
+        nat.check(rc, 'emulate_apply_ptm_direct')
+        assert np.allclose(y, ref, rtol=1e-12, atol=1e-12), (dims, targets, blocks)
+
+
+def test_apply_direct_not_applicable():
+    """Float64, partial tiles or an innermost bystander: QSV_ENOTSUP."""
+    L = nat.lib()
+    P = np.zeros(64 * 64)
+    for dtype, dims, targets in ((nat.QSV_F64, [4] * 6, (3, 4, 5)),
+                                 (nat.QSV_F32, [4] * 5, (2, 3, 4)),
+                                 (nat.QSV_F32, [4] * 6, (1, 2, 3))):
+        x = np.zeros(dims)
+        y = np.zeros(dims)
+        rc = L.qsv_emulate_apply_ptm_direct(dtype, x.ctypes.data_as(nat._c_f64p),
+                                            y.ctypes.data_as(nat._c_f64p), len(dims),
+                                            nat.i64(dims), 3, nat.i32(targets),
+                                            P.ctypes.data_as(nat._c_f64p), 0)
+        assert rc == -3, (dtype, dims, targets)
+
+
 def test_fused_backward_not_applicable():
     """Partial 32-orbit tiles in float32, an innermost target in float64, or
     a lowest PTM index off the innermost axis: QSV_ENOTSUP, callers fall
