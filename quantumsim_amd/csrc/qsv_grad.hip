@@ -537,7 +537,7 @@ typedef float qsv_f2 __attribute__((ext_vector_type(2)));
       : "memory")
 
 // xbar = P^T ybar on the staged tile, wave w's 16 entries 16w..16w+15:
-// x̄^T (32 orbits x 16 entries) = ybar^T (32 x 64) P[:, 16w..16w+15] as two
+// xbar^T (32 orbits x 16 entries) = ybar^T (32 x 64) P[:, 16w..16w+15] as two
 // row tiles, row m of row tile rt being orbit 2m + rt; lane (n, kk) holds
 // B = P[4s + kk][16w + n] for all 16 k-steps in registers (pb).  Slot r of
 // lane l in row tile rt is orbit 8 (l / 16) + 2r + rt of entry 16w + l % 16,
@@ -579,10 +579,11 @@ __device__ __forceinline__ void bwd_apply(const char* lds, const float (&pb)[16]
 // k_ptm_grad64d) and xbar = P^T ybar from the same ybar tile in LDS
 // (bwd_apply), stored from the accumulators: x and ybar are read once, xbar
 // written once (three streams where the transposed apply plus the gradient
-// move four).  The wait before each tile counts the next tile's four loads
-// only: vector-memory loads retire in order among themselves, and the
-// stores issued between them (the previous tile's xbar) can only lower
-// the count, so vmcnt(4) still means this tile's loads are done.
+// move four).  The wait before each tile is vmcnt(4) although the previous
+// tile's xbar stores may be in flight too: vector-memory loads retire in
+// order among themselves, so a load of this tile still in flight would
+// keep the next tile's four later loads in flight as well, five or more in
+// all; at most four outstanding therefore means this tile's loads are done.
 template <bool NT>
 __global__ void __launch_bounds__(kBlock)
     k_bwd64d(const float* __restrict__ x, const float* __restrict__ ybar,
@@ -668,9 +669,9 @@ __global__ void __launch_bounds__(kBlock)
     for (int r = 0; r < 4; ++r) out[(16 * w + 4 * k + r) * 64 + 16 * cb + col] = acc64[cb][r];
 }
 
-// One-pass float32 backward of a dense 64 x 64 block whose targets include
-// the innermost axis (entries in aligned, contiguous runs of four: "quad"
-// geometry).  Tile = 32 orbits; wave-load block ks (0..7) holds orbits
+// One-pass float32 backward of a dense 64 x 64 block whose last target (the
+// PTM's lowest index) is the innermost axis (entries in aligned,
+// contiguous runs of four: "quad" geometry).  Tile = 32 orbits; wave-load block ks (0..7) holds orbits
 // 4ks..4ks+3, its lane l = 4q + okk taking entries 4q..4q+3 of orbit
 // 4ks + okk (16 bytes; when the targets are the last three axes an orbit's
 // 64 entries are one 256-byte run, so a wave-load reads 1 KB contiguous).
