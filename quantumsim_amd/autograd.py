@@ -169,9 +169,10 @@ def _backward_raw(x, gy, ptm, shape, targets, need_ptm_grad, fused=None):
 
     ``fused`` (default: float32 states) uses the one-pass backward of square
     dense 64 x 64 PTMs instead where it applies (qsv_apply_ptm_batched_backward:
-    three state streams instead of four).  For float32 that is the direct-to-LDS
-    kernel, measured on MI355X 15-20 % faster than the two kernels on
-    64 twelve-qubit states; float64 keeps the two kernels by default (its
+    three state streams instead of four).  For float32 those are the
+    direct-to-LDS kernels (innermost axis a bystander, or the PTM's lowest
+    index on it), measured on MI355X 15-36 % faster than the two kernels on
+    8-12-qubit batches; float64 keeps the two kernels by default (its
     one-pass kernel runs at two waves per SIMD and is slower).
     """
     d = [shape[q] for q in targets]
