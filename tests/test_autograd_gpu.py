@@ -508,6 +508,36 @@ def test_fused_backward_matches_separate_kernels(n, targets, dtype, tol):
         assert err < tol, err
 
 
+def test_fused_backward_misaligned_states_fall_back():
+    """States that are not 16-byte aligned (a view one float in) take the
+    two separate kernels, and the geometry stays eligible for aligned
+    states afterwards; both agree with the aligned one-pass result."""
+    from quantumsim_amd.autograd import _backward_raw, _FUSED_GEOMETRY
+    n, targets, B = 7, (1, 2, 3), 2
+    shape = (4,) * n
+    g = torch.Generator().manual_seed(11)
+    N = 4 ** n
+    xs = torch.randn(B * N + 1, generator=g).to(DEV)
+    ys = torch.randn(B * N + 1, generator=g).to(DEV)
+    P = (torch.randn(B, 64, 64, generator=g) / 8).to(DEV)
+    x_al, y_al = xs[:B * N].view(B, N), ys[:B * N].view(B, N)
+    x_mis = xs[1:].view(B, N)
+    y_mis = ys[1:].view(B, N)
+    assert x_mis.data_ptr() % 16 and y_mis.data_ptr() % 16
+    # the same values at both alignments
+    x_mis.copy_(x_al.clone())
+    y_mis.copy_(y_al.clone())
+    x_al = x_mis.clone()
+    y_al = y_mis.clone()
+    gx1, gp1 = _backward_raw(x_mis, y_mis, P, shape, targets, True)
+    assert _FUSED_GEOMETRY.get((torch.float32, shape, targets)) is not False
+    gx2, gp2 = _backward_raw(x_al, y_al, P, shape, targets, True)
+    assert _FUSED_GEOMETRY.get((torch.float32, shape, targets)) is True
+    assert torch.equal(gx1, gx2)
+    err = ((gp1.double() - gp2.double()).abs().max() / gp2.double().abs().max()).item()
+    assert err < 1e-5, err
+
+
 @pytest.mark.parametrize('n,targets', [(6, (3, 4, 5)), (8, (5, 6, 7)), (10, (0, 4, 9)),
                                        (9, (2, 7, 8)), (8, (1, 2, 3))])
 def test_batched_apply_dense64_matches_einsum(n, targets):
