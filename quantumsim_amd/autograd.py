@@ -171,8 +171,8 @@ def _backward_raw(x, gy, ptm, shape, targets, need_ptm_grad, fused=None):
     dense 64 x 64 PTMs instead where it applies (qsv_apply_ptm_batched_backward:
     three state streams instead of four).  For float32 those are the
     direct-to-LDS kernels (innermost axis a bystander, or the PTM's lowest
-    index on it), measured on MI355X 15-36 % faster than the two kernels on
-    8-12-qubit batches; float64 keeps the two kernels by default (its
+    index on it), measured on MI355X at 7-26 % less kernel time than the two
+    kernels on 11-12-qubit batches; float64 keeps the two kernels by default (its
     one-pass kernel runs at two waves per SIMD and is slower).
     """
     d = [shape[q] for q in targets]

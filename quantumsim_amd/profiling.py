@@ -19,6 +19,12 @@ plain copy, then runs the kernels under study.
             adjoint steps
 Both step workloads print the algorithmic bytes of one step by kernel
 family (quantumsim_amd.ledger) before the marker.
+  backward: the backward of one dense 64 x 64 block over --batch float32
+            states (--qubits, --targets): five warm-up rounds of both, the
+            marker, then twice --reps one-pass backwards
+            (qsv_apply_ptm_batched_backward) alternating with --reps of the
+            two kernels it replaces (qsv_ptm_grad and the transposed
+            qsv_apply_ptm_batched)
 """
 import argparse
 import os
@@ -33,7 +39,9 @@ MARKER_ELEMS = 4099
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('what', choices=['apply2q', 'apply1q', 'fused', 'batched', 'diagonal',
-                                     'differentiable'])
+                                     'differentiable', 'backward'])
+    ap.add_argument('--targets', default='5,6,7', help='backward: the block\'s three qubits')
+    ap.add_argument('--batch', type=int, default=64, help='backward: states in the batch')
     ap.add_argument('--qubits', type=int, default=14)
     ap.add_argument('--dtype', default='float64')
     ap.add_argument('--reps', type=int, default=3)
@@ -57,6 +65,53 @@ def main(argv=None):
     del src, dst
     print('copy reference: %d bytes read, %d bytes written' % (N * src_bytes(dt), N * src_bytes(dt)),
           flush=True)
+    if a.what == 'backward':
+        from quantumsim_amd import _native as nat
+        from quantumsim_amd.autograd import _apply_raw, _fused_backward_raw, _ptm_grad_raw
+        tg = tuple(int(t) for t in a.targets.split(','))
+        B, shape = a.batch, (4,) * n
+        g = torch.Generator(device='cuda').manual_seed(7)
+        x = torch.randn(B, N, device='cuda', generator=g)
+        gy = torch.randn(B, N, device='cuda', generator=g)
+        P = torch.randn(B, 64, 64, device='cuda', generator=g) / 8
+        Pt = P.transpose(1, 2).contiguous()
+        gx = torch.empty_like(x)
+        gp = torch.empty(B, 64, 64, device='cuda')
+
+        def one_pass():
+            if _fused_backward_raw(x, gy, P, shape, tg, gx_out=gx, gp_out=gp) is None:
+                raise SystemExit('the one-pass backward does not take this geometry')
+
+        def two_kernels():
+            _ptm_grad_raw(x, gy, shape, tg, [4, 4, 4], out=gp)
+            _apply_raw(gy, Pt, shape, tg, [4, 4, 4], out=gx)
+        # warm-up of both (the device's clocks rise over the first
+        # milliseconds of heavy work: the variant timed first would lose)
+        for _ in range(5):
+            one_pass()
+            two_kernels()
+        torch.cuda.synchronize()
+        print('backward: %d qubits, targets %s, %d float32 states; one pass reads x and '
+              'ybar and writes xbar (%d bytes), the two kernels move 4 N s per state '
+              '(%d bytes)' % (n, tg, B, 3 * B * N * 4, 4 * B * N * 4), flush=True)
+        m = torch.ones(MARKER_ELEMS, dtype=torch.float64, device='cuda')
+        nat.check(nat.lib().qsv_scale(nat.QSV_F64, m.data_ptr(), m.numel(), 1.0,
+                                      torch.cuda.current_stream().cuda_stream), 'scale')
+        torch.cuda.synchronize()
+        ms = []
+        for fn in (one_pass, two_kernels, one_pass, two_kernels):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(a.reps):
+                fn()
+            e1.record()
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1) / a.reps)
+        print('event-timed per call (kernels and launch gaps), alternating: one pass %.3f / '
+              '%.3f ms, two kernels %.3f / %.3f ms' % (ms[0], ms[2], ms[1], ms[3]), flush=True)
+        print('done', flush=True)
+        return 0
     if a.what == 'differentiable':
         from quantumsim_amd.workloads import TrainableCircuit
         from quantumsim_amd import _native as nat
