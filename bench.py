@@ -31,6 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# dense float32 MFMA peak of MI355X (TFLOP/s): the bound of the 64 x 64
+# contractions next to HBM (16-21 flops per byte against a ridge of 19.7)
+MFMA_F32_PEAK_TFLOPS = 157.3
 # share of the free device memory the batched step's kept states may take
 BATCH_BUDGET_FRACTION = 0.6
 
@@ -211,7 +214,7 @@ def bench_batched(args):
     # after it: the two streams would otherwise overlap inside the events)
     led = Ledger(timed=True, isolate=True)
     cb.capture(w, autotune=True, ledger=led)
-    per_kernel = led.summary(HBM_PEAK_GBS)
+    per_kernel = led.summary(HBM_PEAK_GBS, MFMA_F32_PEAK_TFLOPS)
     alg_bytes = led.total_bytes()
     t_setup = time.perf_counter() - t0
     peak_gb = torch.cuda.max_memory_allocated() / 1e9
@@ -309,7 +312,9 @@ def bench_batched(args):
                                 'achieved = bytes / graph-replay step time; per_kernel ms: '
                                 'events around each launch of one eager step, the device idle '
                                 'before and after it; the persistent kernels keep their states '
-                                'in LDS, so their HBM fraction is not their bound'},
+                                'in LDS, so their HBM fraction is not their bound; mfma_frac: '
+                                'the dense families\' MFMA flops / time against the float32 '
+                                'MFMA peak'},
            'dtype': 'f32 state, f64 readout',
            'checkpoint_every': cb.k, 'kept_state_gb': round(cb.kept_bytes(cb.k) / 1e9, 1),
            'checkpoint_budget_gb': round(budget / 1e9, 1),
@@ -402,7 +407,7 @@ def bench_differentiable(args):
     torch.cuda.empty_cache()
     with Ledger(timed=True, isolate=True) as led:
         step()
-    per_kernel = led.summary(HBM_PEAK_GBS)
+    per_kernel = led.summary(HBM_PEAK_GBS, MFMA_F32_PEAK_TFLOPS)
     alg_bytes = led.total_bytes()
     torch.cuda.empty_cache()
     with torch.no_grad():

@@ -81,7 +81,8 @@ def _apply_raw(x, ptm, shape, targets, dims_out, out=None):
     if led:
         dense = ptm.shape[1] == 64 and ptm.shape[2] == 64
         led.end(ev, _ledger.APPLY_DENSE64 if dense else _ledger.APPLY_SMALL,
-                x.element_size() * (B * (x.shape[1] + n_out) + ptm.numel()))
+                x.element_size() * (B * (x.shape[1] + n_out) + ptm.numel()),
+                2 * 64 * B * n_out if dense else 0)
     return y
 
 
@@ -114,7 +115,8 @@ def _ptm_grad_raw(x, ybar, shape, targets, dims_out, out=None):
     if led:
         wide = din > 16 or dout > 16
         led.end(ev, _ledger.GRAD_DENSE64 if wide else _ledger.GRAD_SMALL,
-                x.element_size() * B * (x.shape[1] + ybar.shape[1] + dout * din))
+                x.element_size() * B * (x.shape[1] + ybar.shape[1] + dout * din),
+                2 * B * x.shape[1] * dout if wide else 0)
     return out
 
 
@@ -159,7 +161,8 @@ def _fused_backward_raw(x, gy, ptm, shape, targets, gx_out=None, gp_out=None):
     if led:
         led.end(ev, _ledger.BWD_DENSE64,
                 x.element_size() * (B * (x.shape[1] + gy.shape[1] + gx.shape[1] + 64 * 64)
-                                    + ptm.numel()))
+                                    + ptm.numel()),
+                2 * 64 * B * (x.shape[1] + gx.shape[1]))
     return gx, gp
 
 

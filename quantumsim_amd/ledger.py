@@ -38,11 +38,11 @@ PERSISTENT_ADJ = 'persistent_adjoint'
 
 # the HIP kernels behind each family (rocprofv3 names)
 KERNELS = {
-    APPLY_DENSE64: 'k_fused64 (64 x 64 PTMs on MFMA)',
+    APPLY_DENSE64: 'k_fused64 / k_apply64q (64 x 64 PTMs on MFMA)',
     APPLY_SMALL: 'k_apply_small / k_apply_lds (PTMs up to 16 x 16)',
-    GRAD_DENSE64: 'k_ptm_grad64 + k_ptm_grad_final',
+    GRAD_DENSE64: 'k_ptm_grad64 / k_ptm_grad64d + k_ptm_grad_final',
     GRAD_SMALL: 'k_ptm_grad + k_ptm_grad_final',
-    BWD_DENSE64: 'k_bwd64d + k_ptm_grad_final (fused backward of 64 x 64 blocks)',
+    BWD_DENSE64: 'k_bwd64d / k_bwd64q + k_ptm_grad_final (one-pass backward of 64 x 64 blocks)',
     PERSISTENT_FWD: 'k_circuits_fwd',
     PERSISTENT_ADJ: 'k_circuits_adj',
 }
@@ -61,7 +61,7 @@ class Ledger:
     def __init__(self, timed=True, isolate=False):
         self.timed = timed
         self.isolate = isolate
-        self.rows = {}        # family -> [launches, bytes, [(start, end) events]]
+        self.rows = {}        # family -> [launches, bytes, [(start, end) events], flops]
 
     def __enter__(self):
         global _ACTIVE
@@ -88,11 +88,13 @@ class Ledger:
         ev.record()
         return ev
 
-    def end(self, start, family, nbytes):
-        """Call right after the launch that ``start`` belongs to."""
-        row = self.rows.setdefault(family, [0, 0, []])
+    def end(self, start, family, nbytes, flops=0):
+        """Call right after the launch that ``start`` belongs to; ``flops``:
+        the MFMA contraction's floating-point operations (dense families)."""
+        row = self.rows.setdefault(family, [0, 0, [], 0])
         row[0] += 1
         row[1] += int(nbytes)
+        row[3] += int(flops)
         if start is not None:
             ev = torch.cuda.Event(enable_timing=True)
             ev.record()
@@ -103,17 +105,26 @@ class Ledger:
     def total_bytes(self):
         return sum(r[1] for r in self.rows.values())
 
-    def summary(self, peak_gbs=None):
-        """{family: {launches, bytes, ms, GBs[, frac]}} (waits for the GPU)."""
+    def summary(self, peak_gbs=None, peak_tflops=None):
+        """{family: {launches, bytes, ms, GBs[, frac][, TFLOPs, mfma_frac]}}
+        (waits for the GPU); ``frac`` against ``peak_gbs`` (HBM), and for the
+        families that count flops ``mfma_frac`` against ``peak_tflops``
+        (the dense MFMA peak of the states' element type)."""
         torch.cuda.synchronize()
         out = {}
-        for fam, (n, b, evs) in sorted(self.rows.items()):
+        for fam, (n, b, evs, fl) in sorted(self.rows.items()):
             row = {'kernel': KERNELS.get(fam, fam), 'launches': n, 'bytes': b}
+            if fl:
+                row['flops'] = fl
             if evs:
                 ms = sum(a.elapsed_time(c) for a, c in evs)
                 row['ms'] = round(ms, 3)
                 row['GBs'] = round(b / ms / 1e6, 1) if ms > 0 else None
                 if peak_gbs and row['GBs']:
                     row['frac'] = round(row['GBs'] / peak_gbs, 4)
+                if fl and ms > 0:
+                    row['TFLOPs'] = round(fl / ms / 1e9, 2)
+                    if peak_tflops:
+                        row['mfma_frac'] = round(row['TFLOPs'] / peak_tflops, 4)
             out[fam] = row
         return out
